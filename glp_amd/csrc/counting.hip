@@ -1,0 +1,260 @@
+// Approximate and bounded-memory counting methods, and SLPA post-processing.
+//
+//   count-min  (method/countmin.cuh:10-263, used as a filter by
+//               l_big_update_syn3, header/kernel.cuh:1420-1566): per vertex a
+//               d x W sketch in LDS; every edge adds its frequency to one cell
+//               per row; a label's estimate is the minimum of its cells after
+//               all edges are in; the winner is the packed maximum of
+//               (estimate, label).  Integer sums commute, so the result does
+//               not depend on the order edges are added.  cm_aliased=1 keeps
+//               the rows exactly as countmin.cuh writes them (d=3 adds row 3's
+//               hash into row 2, :185-191; d=4 adds h3 into row 2 twice,
+//               :197-221); cm_aliased=0 uses d independent rows.
+//   frequent   (method/frequent.cuh SharedFA, a Misra-Gries table of C
+//               slots): each vertex's edges are taken in CSR order; a present
+//               label gains its frequency, an absent one takes a free slot,
+//               and with no free slot every counter drops by
+//               min(frequency, smallest counter), emptied slots are freed
+//               and the label enters with what is left (the unit-step
+//               Misra-Gries rule applied frequency times).
+//   buffer     (method/buffer.cuh Buffer): as frequent, except that with no
+//               free slot every counter drops by min(counter, frequency),
+//               emptied slots are freed and the incoming label is dropped
+//               (buffer.cuh:161-172).
+// For frequent and buffer the table is processed sequentially per vertex,
+// the order-dependent semantics the reference's concurrent tables only
+// approximate.  The winner is the largest (count, label) left in the table;
+// a vertex whose table ends empty keeps its label.
+
+#include "device.h"
+#include "device_prims.h"
+#include "session.h"
+
+#include <algorithm>
+#include <memory>
+
+namespace glp {
+using namespace dev;
+
+// Row hashes of method/countmin.cuh:97-142 (the constants must match for
+// count-min results to match).
+__device__ __forceinline__ uint32_t cm_hash(int r, uint32_t k) {
+    switch (r) {
+        case 0:
+            k ^= k >> 16; k *= 0x12ebcb3bu; k ^= k >> 13; k *= 0xc123ae53u; k ^= k >> 16;
+            return k;
+        case 1:
+            k ^= k >> 16; k *= 0x21eb21bbu; k ^= k >> 13; k *= 0x2cb2ea35u; k ^= k >> 16;
+            return k;
+        case 2:
+            k ^= k >> 16; k *= 0x8febce61u; k ^= k >> 13; k *= 0xa2b2be25u; k ^= k >> 16;
+            return k;
+        case 3:
+            k ^= k >> 12; k *= 0x13ebdb3au; k ^= k >> 12; k *= 0xc123ae53u; k ^= k >> 14;
+            return k;
+        default:
+            k ^= k >> 13; k *= 0x312b521bu; k ^= k >> 5; k *= 0x1cb3ea22u; k ^= k >> 17;
+            return k;
+    }
+}
+
+// Cells (row*W + column) a key touches, one entry per increment.
+__device__ __forceinline__ int cm_cells(uint32_t key, int rows, int W, int aliased, uint32_t* cell) {
+    if (!aliased || rows <= 2 || rows == 5) {
+        for (int r = 0; r < rows; ++r) cell[r] = (uint32_t)r * W + cm_hash(r, key) % W;
+        return rows;
+    }
+    cell[0] = cm_hash(0, key) % W;
+    cell[1] = (uint32_t)W + cm_hash(1, key) % W;
+    cell[2] = (uint32_t)W + cm_hash(2, key) % W;  // row 3's hash lands in row 2
+    if (rows == 3) return 3;
+    cell[3] = cell[2];                             // d=4: h3 into row 2 again
+    return 4;
+}
+
+template <bool W>
+__global__ __launch_bounds__(64) void k_countmin(StepArgs a, int32_t rows, int32_t width,
+                                                 int32_t aliased) {
+    extern __shared__ uint32_t cells[];
+    const int lane = lane_id();
+    const int32_t v = (int32_t)blockIdx.x;
+    const int32_t b = a.off[v], e = a.off[v + 1];
+    const int32_t ncell = rows * width;
+    for (int32_t i = lane; i < ncell; i += 64) cells[i] = 0u;
+    __syncthreads();
+    for (int32_t j = b + lane; j < e; j += 64) {
+        const uint32_t key = (uint32_t)a.L[a.nbr[j]] + 1u;
+        const uint32_t f = W ? a.w[j] : 1u;
+        uint32_t c[5];
+        const int k = cm_cells(key, rows, width, aliased, c);
+        for (int i = 0; i < k; ++i) atomicAdd(&cells[c[i]], f);
+    }
+    __syncthreads();
+    uint64_t best = 0ull;
+    for (int32_t j = b + lane; j < e; j += 64) {
+        const uint32_t key = (uint32_t)a.L[a.nbr[j]] + 1u;
+        uint32_t c[5];
+        const int k = cm_cells(key, rows, width, aliased, c);
+        uint32_t est = 0xffffffffu;
+        for (int i = 0; i < k; ++i) est = min(est, cells[c[i]]);
+        best = max_u64(best, pack_score((int32_t)min(est, 0x7fffffffu), key));
+    }
+    best = wave_max_u64(best);
+    if (lane == 0) {
+        const int32_t old = a.keep[v];
+        const int32_t nl = best ? packed_label(best) : old;
+        a.Lw[v] = nl;
+        if (nl != old) atomicAdd(a.changes, 1ull);
+    }
+}
+
+// One lane per vertex; the lane's table lives in LDS (C key and C count
+// slots per lane).
+template <bool W, bool BUF>
+__global__ __launch_bounds__(64) void k_table(StepArgs a, int32_t cap) {
+    extern __shared__ uint32_t tbl[];
+    const int lane = lane_id();
+    const int32_t v = (int32_t)(blockIdx.x * 64 + lane);
+    uint32_t* K = tbl + lane * 2 * cap;
+    uint32_t* C = K + cap;
+    bool changed = false;
+    if (v < a.n) {
+        for (int i = 0; i < cap; ++i) { K[i] = 0u; C[i] = 0u; }
+        const int32_t b = a.off[v], e = a.off[v + 1];
+        for (int32_t j = b; j < e; ++j) {
+            const uint32_t key = (uint32_t)a.L[a.nbr[j]] + 1u;
+            const uint32_t f = W ? a.w[j] : 1u;
+            int hit = -1, fr = -1;
+            uint32_t cmin = 0xffffffffu;
+            for (int i = 0; i < cap; ++i) {
+                const uint32_t k = K[i];
+                if (k == key) hit = i;
+                if (k == 0u) { if (fr < 0) fr = i; }
+                else cmin = min(cmin, C[i]);
+            }
+            if (hit >= 0) { C[hit] += f; continue; }
+            if (fr >= 0) { K[fr] = key; C[fr] = f; continue; }
+            // table full
+            const uint32_t d = min(f, cmin);
+            int freed = -1;
+            for (int i = 0; i < cap; ++i) {
+                const uint32_t c = C[i];
+                const uint32_t nc = BUF ? (c > f ? c - f : 0u) : c - d;
+                C[i] = nc;
+                if (nc == 0u) { K[i] = 0u; if (freed < 0) freed = i; }
+            }
+            if (!BUF && f > d) { K[freed] = key; C[freed] = f - d; }
+        }
+        uint64_t best = 0ull;
+        for (int i = 0; i < cap; ++i)
+            if (K[i]) best = max_u64(best, pack_score((int32_t)min(C[i], 0x7fffffffu), K[i]));
+        const int32_t old = a.keep[v];
+        const int32_t nl = best ? packed_label(best) : old;
+        a.Lw[v] = nl;
+        changed = nl != old;
+    }
+    unsigned long long bal = __ballot(changed);
+    if (lane == 0 && bal) atomicAdd(a.changes, (unsigned long long)__popcll(bal));
+}
+
+int enqueue_counting_step(glp_session* s) {
+    const DeviceGraph& dg = *s->g->dev;
+    StepArgs a{};
+    a.off = dg.off;
+    a.nbr = dg.nbr;
+    a.w = dg.w;
+    a.n = s->n;
+    a.t = s->t + 1;
+    a.changes = s->d_acc;
+    a.L = s->L[s->cur];
+    a.keep = s->L[s->cur];
+    a.Lw = s->L[s->cur ^ 1];
+    const bool w = s->p.weighted != 0;
+    if (s->n > 0) {
+        if (s->p.method == GLP_COUNTMIN) {
+            const size_t lds = (size_t)s->p.cm_rows * s->p.cm_width * 4;
+            if (w) hipLaunchKernelGGL(k_countmin<true>, dim3(s->n), dim3(64), lds, s->stream, a,
+                                      s->p.cm_rows, s->p.cm_width, s->p.cm_aliased);
+            else hipLaunchKernelGGL(k_countmin<false>, dim3(s->n), dim3(64), lds, s->stream, a,
+                                    s->p.cm_rows, s->p.cm_width, s->p.cm_aliased);
+        } else {
+            const size_t lds = (size_t)64 * 2 * s->p.table_cap * 4;
+            const unsigned grid = (unsigned)((s->n + 63) / 64);
+            const bool buf = s->p.method == GLP_BUFFER;
+            if (buf) {
+                if (w) hipLaunchKernelGGL((k_table<true, true>), dim3(grid), dim3(64), lds, s->stream, a, s->p.table_cap);
+                else hipLaunchKernelGGL((k_table<false, true>), dim3(grid), dim3(64), lds, s->stream, a, s->p.table_cap);
+            } else {
+                if (w) hipLaunchKernelGGL((k_table<true, false>), dim3(grid), dim3(64), lds, s->stream, a, s->p.table_cap);
+                else hipLaunchKernelGGL((k_table<false, false>), dim3(grid), dim3(64), lds, s->stream, a, s->p.table_cap);
+            }
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return set_error(GLP_E_HIP, "counting launch: %s", hipGetErrorString(e));
+    }
+    int rc = launch_end_step(s->d_acc, s->d_ring, s->d_ctr, kChangeRing, s->stream);
+    if (rc != GLP_OK) return rc;
+    s->cur ^= 1;
+    s->t += 1;
+    return GLP_OK;
+}
+
+// SLPA post-processing (builder-defined; the reference's SLPA keeps one
+// memory slot and has no threshold): for vertex v the labels whose count in
+// M_v[0..t] is >= r*(t+1), ascending, and the dominant label (highest count,
+// then highest label).
+__global__ __launch_bounds__(64) void k_slpa_post(const int32_t* __restrict__ mem, int32_t n,
+                                                  int32_t rows, int32_t r_num, int32_t r_den,
+                                                  int32_t kmax, int32_t* dominant, int32_t* nsets,
+                                                  int32_t* sets) {
+    const int32_t v = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (v >= n) return;
+    int32_t vals[256];
+    for (int32_t i = 0; i < rows; ++i) vals[i] = mem[(size_t)i * n + v];
+    for (int32_t i = 1; i < rows; ++i) {  // insertion sort, ascending
+        const int32_t x = vals[i];
+        int32_t j = i - 1;
+        while (j >= 0 && vals[j] > x) { vals[j + 1] = vals[j]; --j; }
+        vals[j + 1] = x;
+    }
+    uint64_t best = 0ull;
+    int32_t kept = 0;
+    for (int32_t i = 0; i < rows;) {
+        int32_t j = i;
+        while (j < rows && vals[j] == vals[i]) ++j;
+        const int32_t c = j - i;
+        best = max(best, ((uint64_t)(uint32_t)c << 32) | ((uint32_t)vals[i] + 1u));
+        if ((int64_t)c * r_den >= (int64_t)r_num * rows) {
+            if (kept < kmax) sets[(size_t)v * kmax + kept] = vals[i];
+            ++kept;
+        }
+        i = j;
+    }
+    for (int32_t k = kept; k < kmax; ++k) sets[(size_t)v * kmax + k] = -1;
+    if (dominant) dominant[v] = packed_label(best);
+    if (nsets) nsets[v] = kept;
+}
+
+int slpa_post(glp_session* s, int32_t kmax, int32_t* dominant, int32_t* nsets, int32_t* sets) {
+    const int32_t n = s->n;
+    if (n == 0) return GLP_OK;
+    const int32_t rows = s->t + 1;
+    int32_t *d_dom = nullptr, *d_ns = nullptr, *d_sets = nullptr;
+    struct Guard {
+        int32_t** p[3];
+        ~Guard() { for (auto q : p) (void)hipFree(*q); }
+    } guard{{&d_dom, &d_ns, &d_sets}};
+    GLP_HIP(hipMalloc(&d_dom, (size_t)n * 4));
+    GLP_HIP(hipMalloc(&d_ns, (size_t)n * 4));
+    if (kmax > 0) GLP_HIP(hipMalloc(&d_sets, (size_t)n * kmax * 4));
+    hipLaunchKernelGGL(k_slpa_post, dim3((n + 63) / 64), dim3(64), 0, s->stream, s->mem, n, rows,
+                       s->p.slpa_r_num, s->p.slpa_r_den, kmax, d_dom, d_ns, d_sets);
+    GLP_HIP(hipGetLastError());
+    GLP_HIP(hipStreamSynchronize(s->stream));
+    if (dominant) GLP_HIP(hipMemcpy(dominant, d_dom, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (nsets) GLP_HIP(hipMemcpy(nsets, d_ns, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (kmax > 0 && sets) GLP_HIP(hipMemcpy(sets, d_sets, (size_t)n * kmax * 4, hipMemcpyDeviceToHost));
+    return GLP_OK;
+}
+
+}  // namespace glp

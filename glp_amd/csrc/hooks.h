@@ -1,0 +1,142 @@
+// The four user-facing LP hooks as HIP device functors.
+//
+// The reference specifies them in README.md:72-103 and stubs them as empty
+// kernels in header/kernel.cuh:6071-6085:
+//   PickLabel(vid)                    - decide vid's label for this step
+//   LoadNeighbor(vid, sid) -> (l, f)  - label and frequency an edge contributes
+//   LabelScore(vid, l, f) -> score    - score of candidate label l
+//   UpdateVertex(vid, l, score)       - commit the winner
+// Here a hook set is a struct whose static __device__ members the tier
+// kernels are instantiated over.  Frequencies are exact uint32 sums, and a
+// score is returned already packed into a uint64 whose unsigned order is the
+// lexicographic order (score, label): the high word is the order-preserving
+// image of a signed 32-bit score, the low word is label+1.  The tier kernels
+// take the maximum packed value over all candidate labels, which makes the
+// result independent of the order edges are counted in, and ties go to the
+// highest label id (the packed maximum of header/kernel.cuh:901-903).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace glp {
+
+// Pointers and scalars one step needs; passed by value to every kernel.
+struct StepArgs {
+    const int32_t* off;      // CSR offsets, n+1
+    const int32_t* nbr;      // CSR neighbours, m
+    const uint32_t* w;       // edge weights, m (weighted runs only)
+    const int32_t* L;        // labels neighbours show this step (PickLabel output)
+    const int32_t* adj;      // pre-gathered neighbour labels, m (separate gather)
+    const int32_t* keep;     // label a vertex keeps when it has no neighbours
+    int32_t* Lw;             // labels written by this step
+    const int32_t* Vr;       // layered: volume of each label before the step
+    int32_t* Vw;             // layered: volume after the step (zeroed first)
+    unsigned long long* changes;  // accumulates the number of changed labels
+    int32_t score_mul;       // layered: q+p for gamma = p/q
+    int32_t score_vmul;      // layered: p
+    int32_t n;
+    int32_t t;               // step number (1-based)
+    const int32_t* mem;      // SLPA memory, (T+1) x n
+    uint64_t seed;           // SLPA stream seed
+};
+
+__host__ __device__ __forceinline__ uint32_t order_i32(int32_t s) {
+    return (uint32_t)s ^ 0x80000000u;
+}
+__host__ __device__ __forceinline__ uint64_t pack_score(int32_t score, uint32_t key) {
+    return ((uint64_t)order_i32(score) << 32) | key;
+}
+__host__ __device__ __forceinline__ int32_t packed_label(uint64_t p) {
+    return (int32_t)((uint32_t)p) - 1;
+}
+
+// ---- classic LP: score = frequency (prop/classicLprop.cuh,
+// header/kernel.cuh:1724-1729 packed (count<<32)+key) --------------------
+struct ClassicHooks {
+    static constexpr bool kPick = false;
+    __device__ static __forceinline__ int32_t pick_label(const StepArgs& a, int32_t vid) {
+        return a.keep[vid];
+    }
+    template <bool W, bool G>
+    __device__ static __forceinline__ uint32_t load_neighbor(const StepArgs& a, int32_t vid,
+                                                             int32_t j, uint32_t* freq) {
+        (void)vid;
+        *freq = W ? a.w[j] : 1u;
+        return (uint32_t)(G ? a.adj[j] : a.L[a.nbr[j]]) + 1u;
+    }
+    __device__ static __forceinline__ uint64_t label_score(const StepArgs& a, int32_t vid,
+                                                           uint32_t key, uint32_t freq) {
+        (void)a; (void)vid;
+        // freq < 2^31 (checked on the host), so order_i32 keeps the order
+        return pack_score((int32_t)freq, key);
+    }
+    __device__ static __forceinline__ void update_vertex(const StepArgs& a, int32_t vid,
+                                                         int32_t label, uint64_t score) {
+        (void)score;
+        a.Lw[vid] = label;
+    }
+};
+
+// ---- layered LP: score = (q+p)*C - p*V(l) for gamma = p/q
+// (prop/MultiLprop.cuh:313-314, header/kernel.cuh:2579 delta*count - gcounter)
+struct LayeredHooks {
+    static constexpr bool kPick = false;
+    __device__ static __forceinline__ int32_t pick_label(const StepArgs& a, int32_t vid) {
+        return a.keep[vid];
+    }
+    template <bool W, bool G>
+    __device__ static __forceinline__ uint32_t load_neighbor(const StepArgs& a, int32_t vid,
+                                                             int32_t j, uint32_t* freq) {
+        return ClassicHooks::load_neighbor<W, G>(a, vid, j, freq);
+    }
+    __device__ static __forceinline__ uint64_t label_score(const StepArgs& a, int32_t vid,
+                                                           uint32_t key, uint32_t freq) {
+        (void)vid;
+        // both terms are range-checked on the host to fit int32
+        int32_t s = a.score_mul * (int32_t)freq - a.score_vmul * a.Vr[key - 1];
+        return pack_score(s, key);
+    }
+    __device__ static __forceinline__ void update_vertex(const StepArgs& a, int32_t vid,
+                                                         int32_t label, uint64_t score) {
+        (void)score;
+        a.Lw[vid] = label;
+        atomicAdd(&a.Vw[label], 1);  // header/kernel.cuh:2626 gcounter_temp
+    }
+};
+
+// ---- SLPA listener: classic counting over the speakers' labels; the
+// speaker choice is the pick kernel (slpa_speak).  prop/SLPAprop.cuh:337-398.
+struct SlpaHooks {
+    static constexpr bool kPick = true;
+    // Speaker rule: u shows M_u[R(seed,t,u) mod t] (memory rows 0..t-1).
+    __device__ static __forceinline__ int32_t pick_label(const StepArgs& a, int32_t vid) {
+        uint64_t x = (a.seed << 40) ^ ((uint64_t)(uint32_t)a.t << 32) ^ (uint64_t)(uint32_t)vid;
+        x += 0x9e3779b97f4a7c15ULL;
+        x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+        x ^= x >> 31;
+        uint32_t r = (uint32_t)x;
+        uint32_t row = r % (uint32_t)a.t;
+        return a.mem[(size_t)row * (size_t)a.n + (size_t)vid];
+    }
+    template <bool W, bool G>
+    __device__ static __forceinline__ uint32_t load_neighbor(const StepArgs& a, int32_t vid,
+                                                             int32_t j, uint32_t* freq) {
+        return ClassicHooks::load_neighbor<W, G>(a, vid, j, freq);
+    }
+    __device__ static __forceinline__ uint64_t label_score(const StepArgs& a, int32_t vid,
+                                                           uint32_t key, uint32_t freq) {
+        return ClassicHooks::label_score(a, vid, key, freq);
+    }
+    __device__ static __forceinline__ void update_vertex(const StepArgs& a, int32_t vid,
+                                                         int32_t label, uint64_t score) {
+        (void)score;
+        a.Lw[vid] = label;
+    }
+};
+
+}  // namespace glp
+
+// The user-overridable hook set (method 4).
+#include "user_hooks.h"

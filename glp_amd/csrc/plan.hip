@@ -1,0 +1,137 @@
+// Host tier planner.
+//
+// Replaces the reference's planning in prop/classicLprop.cuh:196-332:
+// binary_search tier boundaries (:129-158, which assume ids sorted by
+// non-increasing degree), the tiny-vertex warp packing (:264-308) and the hub
+// block assignment (compute_num_blocks / exclusive_scan / assign_blocks,
+// header/kernel.cuh:6011-6046).  Here every vertex is classified by its own
+// degree, so no id order is assumed, and low-degree vertices are packed into
+// wavefront groups along runs of consecutive ids (their CSR edges are then
+// contiguous).
+
+#include "device.h"
+#include "device_prims.h"
+
+#include <algorithm>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+
+namespace glp {
+
+Tuning& default_tuning() {
+    static Tuning t;
+    static bool loaded = false;
+    if (!loaded) {
+        loaded = true;
+        // key=value lines written by the autotuner
+        const char* path = getenv("GLP_TUNING");
+        if (path) {
+            std::ifstream in(path);
+            std::string line;
+            while (std::getline(in, line)) {
+                std::istringstream ss(line);
+                std::string k;
+                int v;
+                if (std::getline(ss, k, '=') && (ss >> v)) {
+                    if (k == "pack_max_deg") t.pack_max_deg = v;
+                    else if (k == "wave_max_deg") t.wave_max_deg = v;
+                    else if (k == "block_max_deg") t.block_max_deg = v;
+                    else if (k == "hub_chunk") t.hub_chunk = v;
+                    else if (k == "block_threads") t.block_threads = v;
+                }
+            }
+        }
+    }
+    return t;
+}
+
+static int check_tuning(const Tuning& t) {
+    if (t.pack_max_deg < 0 || t.pack_max_deg > 64) return set_error(GLP_E_ARG, "pack_max_deg must be in [0,64]");
+    if (t.wave_max_deg < t.pack_max_deg || t.wave_max_deg > kWaveCap / 2)
+        return set_error(GLP_E_ARG, "wave_max_deg must be in [pack_max_deg, %d]", kWaveCap / 2);
+    // the largest LDS table (2*deg slots of 8 bytes) must fit a CU's 160 KB
+    if (t.block_max_deg < t.wave_max_deg || t.block_max_deg > 8192)
+        return set_error(GLP_E_ARG, "block_max_deg must be in [wave_max_deg, 8192]");
+    if (t.hub_chunk < 256 || t.hub_chunk > kHubLdsCap / 2 || (t.hub_chunk & 255))
+        return set_error(GLP_E_ARG, "hub_chunk must be a multiple of 256 in [256, %d]", kHubLdsCap / 2);
+    if (t.block_threads != 256 && t.block_threads != 512 && t.block_threads != 1024)
+        return set_error(GLP_E_ARG, "block_threads must be 256, 512 or 1024");
+    return GLP_OK;
+}
+
+int build_plan(const HostCSR& g, const Tuning& t, Plan& p) {
+    int rc = check_tuning(t);
+    if (rc != GLP_OK) return rc;
+    p = Plan();
+    p.tune = t;
+    // block classes: LDS table capacity 2*deg rounded up to a power of two
+    {
+        uint32_t lo = dev::next_pow2(2u * (uint32_t)std::max(t.wave_max_deg + 1, 1));
+        uint32_t hi = dev::next_pow2(2u * (uint32_t)std::max(t.block_max_deg, 1));
+        lo = std::max(lo, 64u);
+        int c = 0;
+        // classes double in size; the last absorbs everything up to hi
+        uint32_t cap = lo;
+        while (c < kMaxBlockClasses) {
+            p.class_cap[c++] = (int32_t)cap;
+            if (cap >= hi) break;
+            cap = (c == kMaxBlockClasses - 1) ? hi : cap * 2;
+        }
+        p.nclasses = t.block_max_deg > t.wave_max_deg ? c : 0;
+    }
+    const int32_t n = g.n;
+    int32_t gs = -1, gc = 0, ge = 0;  // open packed group: start, vertex count, edges
+    auto close_group = [&]() {
+        if (gs >= 0) p.groups.push_back(make_int2(gs, gc));
+        gs = -1;
+        gc = 0;
+        ge = 0;
+    };
+    int64_t hub_chunks = 0;
+    for (int32_t v = 0; v < n; ++v) {
+        const int32_t d = g.offsets[v + 1] - g.offsets[v];
+        if (d <= t.pack_max_deg) {
+            if (gs >= 0 && (gc == 64 || ge + d > 64)) close_group();
+            if (gs < 0) gs = v;
+            ++gc;
+            ge += d;
+            p.edges[0] += d;
+            p.verts[0]++;
+            continue;
+        }
+        close_group();
+        if (d <= t.wave_max_deg) {
+            p.wave.push_back(v);
+            p.edges[1] += d;
+            p.verts[1]++;
+        } else if (d <= t.block_max_deg) {
+            const uint32_t need = std::max(dev::next_pow2(2u * (uint32_t)d), 64u);
+            int c = 0;
+            while (c < p.nclasses - 1 && (uint32_t)p.class_cap[c] < need) ++c;
+            p.block[c].push_back(v);
+            p.edges[2] += d;
+            p.verts[2]++;
+        } else {
+            const int32_t h = (int32_t)p.hubs.size();
+            p.hubs.push_back(v);
+            const int32_t nch = (d + t.hub_chunk - 1) / t.hub_chunk;
+            for (int32_t c = 0; c < nch; ++c) p.hub_chunks.push_back(make_int2(h, c));
+            hub_chunks += nch;
+            const uint32_t distinct = (uint32_t)std::min<int64_t>(d, n);
+            const uint32_t cap = std::max(dev::next_pow2(2u * distinct), 64u);
+            p.hub_tbl_off.push_back(p.tbl_slots);
+            p.hub_cap.push_back(cap);
+            p.hub_occ_off.push_back(p.occ_slots);
+            p.tbl_slots += cap;
+            p.occ_slots += distinct;
+            p.edges[3] += d;
+            p.verts[3]++;
+        }
+    }
+    close_group();
+    if (hub_chunks > INT32_MAX) return set_error(GLP_E_RANGE, "too many hub chunks");
+    return GLP_OK;
+}
+
+}  // namespace glp

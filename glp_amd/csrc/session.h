@@ -1,0 +1,57 @@
+// Session state shared by the session, counting, batch and server units.
+#pragma once
+
+#include "device.h"
+
+#include <array>
+#include <vector>
+
+namespace glp {
+constexpr int32_t kChangeRing = 4096;  // per-step change counts kept on the device
+}
+
+struct glp_session {
+    glp_graph* g = nullptr;
+    glp_params p{};
+    glp::Method method = glp::Method::Classic;
+    int32_t n = 0;
+    hipStream_t stream = nullptr;
+    int32_t* L[2] = {nullptr, nullptr};  // labels, double-buffered
+    int32_t* V[2] = {nullptr, nullptr};  // layered volume histogram, double-buffered
+    int32_t* shown = nullptr;            // PickLabel output (SLPA speakers, user hooks)
+    int32_t* adj = nullptr;              // pre-gathered neighbour labels (gather mode)
+    int32_t* mem = nullptr;              // SLPA memory (T+1) x n
+    void* cm_scratch = nullptr;          // counting-method scratch
+    glp::HubScratch hs;
+    unsigned long long* d_acc = nullptr;   // change counter of the running step
+    unsigned long long* d_ring = nullptr;  // change count of each finished step
+    int32_t* d_ctr = nullptr;              // device-side step counter
+    int cur = 0;                           // index of the current label buffer
+    int32_t t = 0;                         // steps enqueued
+    int32_t pending_steps = 0;
+    size_t bytes = 0;                      // device bytes owned (graph excluded)
+    std::vector<std::array<hipEvent_t, 2 * GLP_NTIERS>> ev_pool;  // profiled steps' events
+    size_t ev_used = 0;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    glp_timing timing{};
+    std::vector<int64_t> host_changes;
+    ~glp_session();
+};
+
+namespace glp {
+int ensure_device();
+int check_params(const HostCSR& g, const glp_params* p);
+size_t session_bytes(const HostCSR& g, const Plan& p, const glp_params& prm);
+int session_create(glp_graph* g, const glp_params* p, glp_session** out);
+int session_reset(glp_session* s);
+int session_step(glp_session* s, int32_t steps);
+int session_step_graph(glp_session* s, int32_t steps);
+int session_sync(glp_session* s);
+int session_labels(glp_session* s, int32_t* out);
+int enqueue_step(glp_session* s, hipEvent_t* ev);
+// count-min / frequent / buffer steps (counting.hip)
+int enqueue_counting_step(glp_session* s);
+// SLPA post-processing (counting.hip)
+int slpa_post(glp_session* s, int32_t kmax, int32_t* dominant, int32_t* nsets, int32_t* sets);
+}  // namespace glp

@@ -1,0 +1,207 @@
+/*
+ * glp.h - C ABI of libglp.so, the MI355X (gfx950) label-propagation library.
+ *
+ * Every entry point takes plain pointers and sizes, returns GLP_OK (0) or a
+ * negative GLP_E_* code, and leaves a thread-local message behind for
+ * glp_last_error().  The caller owns every host buffer it passes; the library
+ * copies inputs and writes outputs.  Nothing here allocates on behalf of the
+ * caller except the opaque handles, which the matching *_free releases.
+ *
+ * Each declaration names the reference interface (Gibyeng/GLP) it replaces.
+ */
+#ifndef GLP_H
+#define GLP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ----------------------------------------------------- */
+enum {
+    GLP_OK = 0,
+    GLP_E_ARG = -1,         /* bad argument (null pointer, out-of-range field) */
+    GLP_E_IO = -2,          /* file missing / unreadable / unwritable          */
+    GLP_E_FORMAT = -3,      /* malformed input (negative id, odd-length file)  */
+    GLP_E_HIP = -4,         /* a HIP runtime call failed                       */
+    GLP_E_NOMEM = -5,       /* host or device allocation failed                */
+    GLP_E_RANGE = -6,       /* sizes beyond int32 CSR / 32-bit score limits    */
+    GLP_E_UNSUPPORTED = -7, /* parameter combination not implemented           */
+    GLP_E_BUDGET = -8,      /* job larger than the server's HBM budget         */
+    GLP_E_STATE = -9,       /* call not valid in the handle's current state    */
+    GLP_E_NODEVICE = -10    /* no HIP device visible                           */
+};
+
+/* ---- LP methods (reference main.cu:63-101 "case 1..4", plus counting kinds) */
+enum {
+    GLP_CLASSIC = 1,   /* prop/classicLprop.cuh                                    */
+    GLP_LAYERED = 2,   /* prop/MultiLprop.cuh (score (1+gamma)C - gamma*V)          */
+    GLP_SLPA = 3,      /* prop/SLPAprop.cuh (speaker/listener memory)              */
+    GLP_USER = 4,      /* prop/UserdefinedLprop.cuh: the compiled-in user hooks    */
+    GLP_COUNTMIN = 5,  /* method/countmin.cuh SharedCM sketch counting             */
+    GLP_FREQUENT = 6,  /* method/frequent.cuh SharedFA heavy-hitter table          */
+    GLP_BUFFER = 7     /* method/buffer.cuh Buffer decrement-by-x table            */
+};
+
+/* ---- opaque handles --------------------------------------------------- */
+typedef struct glp_graph glp_graph;       /* host CSR (+ device copy once used) */
+typedef struct glp_session glp_session;   /* device-resident LP state of one job */
+typedef struct glp_batch glp_batch;       /* many graphs, one persistent launch  */
+typedef struct glp_server glp_server;     /* continuous-batching request loop    */
+
+/* ---- run parameters --------------------------------------------------- */
+typedef struct glp_params {
+    int32_t method;       /* GLP_CLASSIC ... GLP_BUFFER                          */
+    int32_t weighted;     /* 1: frequency = sum of uint32 edge weights           */
+    int32_t gamma_num;    /* layered: gamma = gamma_num / gamma_den,             */
+    int32_t gamma_den;    /*   gamma_den a power of two (dyadic, exact scores)   */
+    uint64_t seed;        /* SLPA speaker-stream seed                            */
+    int32_t slpa_T;       /* SLPA memory length = number of listener steps       */
+    int32_t slpa_r_num;   /* SLPA post-processing threshold r = num / den        */
+    int32_t slpa_r_den;
+    int32_t cm_rows;      /* count-min rows d (1..5)                             */
+    int32_t cm_width;     /* count-min row width W (1..4096)                     */
+    int32_t cm_aliased;   /* 1: rows 2 and 3 share storage as countmin.cuh:190   */
+    int32_t table_cap;    /* frequent / buffer table capacity C (1..64)          */
+    int32_t gather;       /* 0: gather fused into counting; 1: separate gather   */
+    int32_t profile;      /* 1: per-tier HIP-event timing inside each step       */
+    int32_t reserved[6];  /* must be zero                                        */
+} glp_params;
+
+/* Per-tier timing and shape record.  Replaces header/test_result.h:5-23
+ * (Test_result): times are milliseconds from HIP events, not host clocks. */
+enum { GLP_TIER_PACKED = 0, GLP_TIER_WAVE = 1, GLP_TIER_BLOCK = 2, GLP_TIER_HUB = 3,
+       GLP_TIER_GATHER = 4, GLP_TIER_PICK = 5, GLP_TIER_AUX = 6, GLP_NTIERS = 8 };
+typedef struct glp_timing {
+    double upload_ms;                /* H2D of CSR (load_graph_to_gpu)              */
+    double plan_ms;                  /* host tier planning (other_proprocess)       */
+    double steps_ms;                 /* sum over executed steps (lp)                */
+    double tier_ms[GLP_NTIERS];      /* per-tier sums when params.profile = 1       */
+    int64_t tier_launches[GLP_NTIERS];
+    int64_t tier_vertices[GLP_NTIERS];
+    int64_t tier_edges[GLP_NTIERS];
+    int32_t steps;                   /* steps executed so far                       */
+    int32_t reserved;
+} glp_timing;
+
+/* ---- library ---------------------------------------------------------- */
+const char* glp_last_error(void);                 /* thread-local message          */
+const char* glp_version(void);
+void glp_params_default(glp_params* p);           /* classic, unweighted, gamma 0  */
+int glp_device_count(int32_t* count);
+
+/* ---- graphs ----------------------------------------------------------- */
+/* Replaces header/graph.h:69-147 (CSRGraph ctor) with :227-258
+ * (load_edgelist_bin): headerless little-endian int32 (u,v) pairs, trailing
+ * partial pair dropped, self-loops dropped, symmetrized unless directed,
+ * duplicates and file order kept.  weighted=1 reads (u,v,w) int32 triples
+ * (no reference counterpart: README.md:54-63 names weighted input but no
+ * binary form exists).  Negative ids are GLP_E_FORMAT.                       */
+int glp_graph_load_bin(const char* path, int32_t directed, int32_t weighted,
+                       glp_graph** out);
+/* Builds a graph from caller CSR arrays (copied).  weights may be NULL.
+ * Replaces the CSRGraph members offsets/neighbors (header/graph.h:195-196).  */
+int glp_graph_from_csr(int32_t n, int64_t m, const int32_t* offsets,
+                       const int32_t* neighbors, const uint32_t* weights,
+                       glp_graph** out);
+/* Shape of a graph: n (CSRGraph::n), m (CSRGraph::m), max degree, weighted. */
+int glp_graph_info(const glp_graph* g, int32_t* n, int64_t* m,
+                   int32_t* max_degree, int32_t* weighted);
+/* Copies the host CSR out (offsets n+1, neighbors m, weights m or NULL).    */
+int glp_graph_csr(const glp_graph* g, int32_t* offsets, int32_t* neighbors,
+                  uint32_t* weights);
+/* Replaces header/graph.h:149 (~CSRGraph); also frees the device copy.      */
+void glp_graph_free(glp_graph* g);
+
+/* ---- one-shot run ----------------------------------------------------- */
+/* Replaces classicLprop/MultiLprop/SLPAprop/UserdefinedLprop::run(niter)
+ * (prop/classicLprop.cuh:454-462, prop/MultiLprop.cuh:383-391,
+ * prop/SLPAprop.cuh:407, main.cu:63-101).  Executes exactly `steps`
+ * propagation steps from L_0[v] = v (the reference CLI runs niter-1 steps,
+ * prop/classicLprop.cuh:361; the CLI applies that convention itself).
+ * labels_out: n int32 (final labels; for SLPA the dominant label).
+ * step_dump: steps*n int32 (labels after each step) or NULL.
+ * timing: may be NULL.                                                       */
+int glp_run(glp_graph* g, const glp_params* p, int32_t steps,
+            int32_t* labels_out, int32_t* step_dump, glp_timing* timing);
+
+/* ---- sessions: device-resident state for repeated steps --------------- */
+/* Uploads the CSR (once per graph), plans tiers and allocates label state;
+ * labels start at L_0[v] = v (header/kernel.cuh:444-451 initialize_labels). */
+int glp_session_create(glp_graph* g, const glp_params* p, glp_session** out);
+/* Resets labels to L_0 and the step counter to 0.                           */
+int glp_session_reset(glp_session* s);
+/* Enqueues `steps` steps on the session's stream (asynchronous).  One step is
+ * the body of the reference loop prop/classicLprop.cuh:361-443: PickLabel,
+ * LoadNeighbor + count per tier, LabelScore, UpdateVertex.                    */
+int glp_session_step(glp_session* s, int32_t steps);
+/* Captures one step in a HIP graph and replays it `steps` times.            */
+int glp_session_step_graph(glp_session* s, int32_t steps);
+/* Waits for the session's stream.                                           */
+int glp_session_sync(glp_session* s);
+/* Copies the current labels (n int32) to host; blocks.                      */
+int glp_session_labels(glp_session* s, int32_t* out);
+/* Number of labels changed by each executed step (the reference's
+ * "node update" counter, prop/classicLprop.cuh:438-440); fills up to cap.   */
+int glp_session_changes(glp_session* s, int64_t* out, int32_t cap,
+                        int32_t* nsteps);
+/* Timing record (Test_result replacement).                                  */
+int glp_session_timing(glp_session* s, glp_timing* out);
+/* Layered: current volume histogram V(l) (n int32).                         */
+int glp_session_volume(glp_session* s, int32_t* out);
+/* Checkpoint / resume: step counter plus label (and volume / SLPA memory)
+ * state.  state_bytes reports the size needed; state may be NULL to query.  */
+int glp_session_save(glp_session* s, void* state, size_t* state_bytes);
+int glp_session_load(glp_session* s, const void* state, size_t state_bytes);
+/* SLPA: memory rows M[0..t] (t+1 rows of n int32, row-major by step).       */
+int glp_session_slpa_memory(glp_session* s, int32_t* out);
+/* SLPA post-processing (threshold r): per vertex the labels whose frequency
+ * in M_v[0..T] is >= r*(T+1), ascending, padded with -1 to kmax, the count
+ * of such labels, and the dominant label (highest count, then highest id).  */
+int glp_session_slpa_post(glp_session* s, int32_t kmax, int32_t* dominant,
+                          int32_t* nsets, int32_t* sets);
+/* Ids of the session's hot kernels and their HIP-event durations (ms) over
+ * the last profiled steps, for roofline accounting.                          */
+int glp_session_free(glp_session* s);
+
+/* ---- batched: many graphs in one persistent launch --------------------- */
+/* No single reference counterpart: the reference runs one graph per process
+ * (main.cu:18-123).  All graphs use the same params; labels are identical to
+ * running each graph alone.                                                   */
+int glp_batch_create(int32_t ngraphs, glp_graph* const* graphs,
+                     const glp_params* p, glp_batch** out);
+/* mode 0: one persistent launch per step; mode 1: steps captured in a HIP
+ * graph; mode 2: plain per-graph session launches (the comparison form).    */
+int glp_batch_run(glp_batch* b, int32_t steps, int32_t mode, double* ms);
+int glp_batch_labels(glp_batch* b, int32_t graph, int32_t* out);
+int glp_batch_reset(glp_batch* b);
+void glp_batch_free(glp_batch* b);
+
+/* ---- serving: continuous batching under an HBM budget ------------------ */
+typedef struct glp_job_stats {
+    double submit_ms;    /* host clock at submit, relative to server start      */
+    double admit_ms;     /* when its memory was reserved and state uploaded     */
+    double done_ms;      /* when its last step completed                        */
+    int64_t bytes;       /* device footprint admitted against the budget        */
+    int32_t steps;
+    int32_t state;       /* 0 queued, 1 running, 2 done, 3 cancelled, <0 error  */
+} glp_job_stats;
+int glp_server_create(uint64_t hbm_budget_bytes, int32_t max_slots,
+                      glp_server** out);
+/* Queues a job (graph is copied into the server; the caller may free it).   */
+int glp_server_submit(glp_server* sv, const glp_graph* g, const glp_params* p,
+                      int32_t steps, int64_t* job_id);
+int glp_server_poll(glp_server* sv, int64_t job_id, glp_job_stats* stats);
+/* Blocks until the job finishes, then copies its labels (n int32).          */
+int glp_server_wait(glp_server* sv, int64_t job_id, int32_t* labels_out,
+                    glp_job_stats* stats);
+int glp_server_cancel(glp_server* sv, int64_t job_id);
+/* Drains all queued jobs and stops the request loop.                        */
+int glp_server_free(glp_server* sv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GLP_H */

@@ -1,0 +1,164 @@
+"""GPU parity: the HIP path (libglp.so through its C ABI) against the CPU
+oracle, bit-exact at every step."""
+import numpy as np
+import pytest
+
+import glp_amd as G
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_steps(graph, steps, **kw):
+    _, dump, _ = G.run(graph, steps, dump=True, **kw)
+    return dump
+
+
+def hand_graph(edges, n, weights=None):
+    off, nbr, w = O.csr_from_edges(n, edges, weights)
+    return G.Graph.from_csr(off, nbr, w), off, nbr, w
+
+
+def assert_steps_equal(got, want):
+    for t in range(len(want)):
+        bad = np.flatnonzero(got[t] != want[t])
+        assert len(bad) == 0, "step %d: %d labels differ, first v=%d gpu=%d oracle=%d" % (
+            t + 1, len(bad), bad[0], got[t][bad[0]], want[t][bad[0]])
+
+
+# ------------------------------------------------------------------ classic ----
+
+def test_classic_closed_forms():
+    g, off, nbr, _ = hand_graph([(0, 1), (1, 2), (0, 2), (3, 4), (4, 5), (3, 5), (2, 3)], 7)
+    got = gpu_steps(g, 2)
+    assert got[0].tolist() == [2, 2, 3, 5, 5, 4, 6]
+    assert got[1].tolist() == [3, 3, 2, 5, 5, 5, 6]
+    star, *_ = hand_graph([(0, i) for i in range(1, 6)], 6)
+    got = gpu_steps(star, 2)
+    assert got[0].tolist() == [5, 0, 0, 0, 0, 0] and got[1].tolist() == [0, 5, 5, 5, 5, 5]
+
+
+def test_classic_planted_65536_twenty_steps():
+    g = G.Graph.planted(65536, 64, 12.0, 4.0, seed=1)
+    off, nbr, _ = g.csr()
+    got = gpu_steps(g, 20)
+    assert_steps_equal(got, O.classic(off, nbr, 20))
+
+
+@pytest.mark.parametrize("scale,ef,sort", [(14, 16, True), (16, 16, True), (15, 8, False)])
+def test_classic_rmat_all_tiers(scale, ef, sort):
+    g = G.Graph.rmat(scale, ef, seed=scale * 7 + ef, degree_sort=sort)
+    off, nbr, _ = g.csr()
+    labels, dump, timing = G.run(g, 6, dump=True)
+    assert_steps_equal(dump, O.classic(off, nbr, 6))
+    if scale == 16:  # max degree ~1e4: packed, wave, block and hub tiers all run
+        for tier in ("packed", "wave", "block", "hub"):
+            assert timing["tier_vertices"][tier] > 0, tier
+
+
+def tier_boundary_graph(seed=0):
+    """Centres with degrees on every tier and hub-chunk boundary, attached to a
+    ring of small communities so neighbour labels repeat and tie."""
+    rng = np.random.default_rng(seed)
+    degs = [1, 2, 31, 32, 33, 63, 64, 65, 255, 256, 257, 511, 512, 513, 2047, 2048, 2049,
+            4095, 4096, 4097, 8191, 8192, 8193, 12289]
+    base = len(degs)
+    pool = 20000
+    n = base + pool
+    edges = []
+    for i, d in enumerate(degs):
+        nb = rng.choice(pool, size=d, replace=False) + base
+        edges += [(i, int(x)) for x in nb]
+    # background: pool vertices in communities of 50 with sparse ties
+    for c in range(pool // 50):
+        members = base + c * 50 + np.arange(50)
+        for _ in range(120):
+            a, b = rng.choice(members, size=2, replace=False)
+            edges.append((int(a), int(b)))
+    return n, edges
+
+
+def test_classic_tier_boundaries():
+    n, edges = tier_boundary_graph()
+    g, off, nbr, _ = hand_graph(edges, n)
+    assert_steps_equal(gpu_steps(g, 5), O.classic(off, nbr, 5))
+
+
+def test_classic_multigraph_and_isolated():
+    rng = np.random.default_rng(1)
+    n = 3000
+    e = rng.integers(0, n // 2, size=(20000, 2))          # upper half isolated
+    e = np.concatenate([e, e[:5000]])                      # duplicates
+    g, off, nbr, _ = hand_graph([tuple(x) for x in e], n)
+    assert_steps_equal(gpu_steps(g, 4), O.classic(off, nbr, 4))
+
+
+def test_classic_weighted():
+    g = G.Graph.rmat(13, 16, seed=5, wmax=8)
+    off, nbr, w = g.csr()
+    assert_steps_equal(gpu_steps(g, 5, weighted=True), O.classic(off, nbr, 5, w=w))
+
+
+def test_neighbour_order_invariance():
+    g = G.Graph.rmat(13, 16, seed=9)
+    off, nbr, _ = g.csr()
+    rng = np.random.default_rng(0)
+    nbr2 = nbr.copy()
+    for v in range(len(off) - 1):
+        rng.shuffle(nbr2[off[v]:off[v + 1]])
+    g2 = G.Graph.from_csr(off, nbr2)
+    assert np.array_equal(gpu_steps(g, 4), gpu_steps(g2, 4))
+
+
+def test_empty_and_edgeless():
+    g = G.Graph.from_csr(np.zeros(1, np.int32), np.zeros(0, np.int32))
+    labels, _ = G.run(g, 3)
+    assert labels.shape == (0,)
+    g = G.Graph.from_csr(np.zeros(11, np.int32), np.zeros(0, np.int32))
+    labels, _ = G.run(g, 3)
+    assert labels.tolist() == list(range(10))
+
+
+def test_separate_gather_matches_fused():
+    g = G.Graph.rmat(15, 16, seed=3)
+    off, nbr, _ = g.csr()
+    want = O.classic(off, nbr, 5)
+    assert_steps_equal(gpu_steps(g, 5, gather=True), want)
+
+
+def test_graph_replay_matches_eager_and_changes():
+    g = G.Graph.rmat(14, 16, seed=4)
+    off, nbr, _ = g.csr()
+    want = O.classic(off, nbr, 7)
+    with G.Session(g) as s:
+        s.step(3)
+        s.sync()
+        assert np.array_equal(s.labels(), want[2])
+        for k in range(4):                      # replays with eager syncs between
+            s.step(1, graph=True)
+            s.sync()
+            assert np.array_equal(s.labels(), want[3 + k])
+        ch = s.changes()
+    prev = np.vstack([np.arange(len(off) - 1, dtype=np.int32), want[:-1]])
+    assert ch.tolist() == [int((want[t] != prev[t]).sum()) for t in range(7)]
+
+
+def test_checkpoint_resume():
+    g = G.Graph.rmat(13, 16, seed=6)
+    off, nbr, _ = g.csr()
+    want = O.classic(off, nbr, 6)
+    with G.Session(g) as s:
+        s.step(3)
+        state = s.save()
+    with G.Session(g) as s2:
+        s2.load(state)
+        s2.step(3)
+        assert np.array_equal(s2.labels(), want[5])
+
+
+def test_profile_mode_matches():
+    g = G.Graph.rmat(14, 16, seed=8)
+    off, nbr, _ = g.csr()
+    labels, t = G.run(g, 3, profile=True)
+    assert np.array_equal(labels, O.classic(off, nbr, 3)[-1])
+    assert t["tier_launches"]["packed"] == 3
