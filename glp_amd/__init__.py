@@ -136,9 +136,44 @@ _sig("glp_server_cancel", ctypes.c_int, _vp, ctypes.c_int64)
 _sig("glp_server_free", ctypes.c_int, _vp)
 
 
+_sig("glp_tuning_set", ctypes.c_int, ctypes.c_char_p, ctypes.c_int32)
+_sig("glp_tuning_get", ctypes.c_int, ctypes.c_char_p, _c_i32p)
+
+TUNING_KEYS = ("pack_max_deg", "wave_max_deg", "block_max_deg", "hub_chunk", "part_keys",
+               "part_limit", "block_threads")
+
+
 def lib():
     """The loaded libglp.so handle."""
     return _lib
+
+
+def tuning_get(key):
+    v = ctypes.c_int32()
+    _check(_lib.glp_tuning_get(key.encode(), ctypes.byref(v)))
+    return v.value
+
+
+def tuning_set(key, value):
+    _check(_lib.glp_tuning_set(key.encode(), int(value)))
+
+
+class tuning:
+    """Context manager: process-wide tuning for graphs uploaded inside it."""
+
+    def __init__(self, **kw):
+        self.kw = kw
+        self.saved = {}
+
+    def __enter__(self):
+        self.saved = {k: tuning_get(k) for k in TUNING_KEYS}
+        for k, v in self.kw.items():
+            tuning_set(k, v)
+        return self
+
+    def __exit__(self, *a):
+        for k, v in self.saved.items():
+            tuning_set(k, v)
 
 
 def _check(rc):

@@ -104,7 +104,7 @@ __global__ __launch_bounds__(64) void k_countmin(StepArgs a, int32_t rows, int32
         const int32_t old = a.keep[v];
         const int32_t nl = best ? packed_label(best) : old;
         a.Lw[v] = nl;
-        if (nl != old) atomicAdd(a.changes, 1ull);
+        if (nl != old) atomicAdd(&a.changes[blockIdx.x & (kChangeSlots - 1)], 1ull);
     }
 }
 
@@ -154,7 +154,8 @@ __global__ __launch_bounds__(64) void k_table(StepArgs a, int32_t cap) {
         changed = nl != old;
     }
     unsigned long long bal = __ballot(changed);
-    if (lane == 0 && bal) atomicAdd(a.changes, (unsigned long long)__popcll(bal));
+    if (lane == 0 && bal)
+        atomicAdd(&a.changes[blockIdx.x & (kChangeSlots - 1)], (unsigned long long)__popcll(bal));
 }
 
 int enqueue_counting_step(glp_session* s) {

@@ -23,13 +23,17 @@ struct Tuning {
     int32_t pack_max_deg = 32;     // deg <= this: lane-packed groups of <= 64 edges
     int32_t wave_max_deg = 256;    // deg <= this: one wavefront per vertex (LDS table)
     int32_t block_max_deg = 4096;  // deg <= this: one workgroup per vertex (LDS table)
-    int32_t hub_chunk = 4096;      // edges per workgroup for hubs (deg > block_max_deg)
+    int32_t hub_chunk = 2048;      // edges per aggregation workgroup for hubs (deg > block_max_deg)
+    int32_t part_keys = 2048;      // expected distinct labels per hub counting workgroup
+    int32_t part_limit = 0;        // distinct labels before a hub range is halved (0: table/2)
     int32_t block_threads = 256;   // workgroup size of the block tier
 };
+int tuning_set(Tuning& t, const char* key, int32_t value);  // GLP_E_ARG on unknown key
+int tuning_get(const Tuning& t, const char* key, int32_t* value);
 Tuning& default_tuning();  // process-wide defaults (GLP_TUNING env file overrides)
 
+constexpr int kChangeSlots = 256;  // change counters, spread to avoid one hot address
 constexpr int kWaveCap = 512;      // LDS slots per wavefront in the wave tier
-constexpr int kHubLdsCap = 8192;   // LDS slots per workgroup in the hub count pass
 constexpr int kMaxBlockClasses = 5;
 
 // Host-built tier plan.  Vertex lists hold ids; groups hold (first id, count).
@@ -41,11 +45,11 @@ struct Plan {
     int32_t class_cap[kMaxBlockClasses] = {};  // LDS slots of each block class
     std::vector<int32_t> block[kMaxBlockClasses];
     std::vector<int32_t> hubs;                // hub tier
-    std::vector<int2> hub_chunks;             // (hub index, chunk index)
-    std::vector<int64_t> hub_tbl_off;         // slot offset of each hub's HBM table
-    std::vector<uint32_t> hub_cap;            // table slots (power of two)
-    std::vector<int64_t> hub_occ_off;         // offset of each hub's occupied-slot list
-    int64_t tbl_slots = 0, occ_slots = 0;
+    std::vector<int2> hub_chunks;             // (hub index, chunk index): aggregation workgroups
+    std::vector<int32_t> hub_chunk0;          // index of each hub's first chunk
+    std::vector<int64_t> hub_base;            // first pair slot of each hub (chunk regions)
+    std::vector<int2> hub_parts;              // (hub index, part index): counting workgroups
+    int64_t pair_slots = 0;                   // sum over hubs of chunks * hub_chunk
     int64_t edges[4] = {}, verts[4] = {};     // per tier: packed, wave, block, hub
 };
 int build_plan(const HostCSR& g, const Tuning& t, Plan& p);
@@ -63,24 +67,55 @@ struct DeviceGraph {
     int32_t* d_block[kMaxBlockClasses] = {};
     int32_t* d_hubs = nullptr;
     int2* d_hub_chunks = nullptr;
-    int64_t* d_hub_tbl_off = nullptr;
-    uint32_t* d_hub_cap = nullptr;
-    int64_t* d_hub_occ_off = nullptr;
+    int32_t* d_hub_chunk0 = nullptr;
+    int64_t* d_hub_base = nullptr;
+    int2* d_hub_parts = nullptr;
     double upload_ms = 0, plan_ms = 0;
     size_t bytes = 0;
     ~DeviceGraph();
 };
 int device_graph_create(const HostCSR& g, const Tuning& t, hipStream_t s, DeviceGraph** out);
+// Uploads a graph with an already built plan (synchronises stream s).
+int device_graph_upload(const HostCSR& g, Plan&& plan, hipStream_t s, DeviceGraph** out);
 // HBM bytes a device graph of this shape needs (planning estimate).
 size_t device_graph_bytes(const HostCSR& g, const Plan& p);
 
-// Per-session hub scratch: HBM hash tables (zero between steps), occupied
-// lists, per-hub counters and winners.
+// Per-session hub scratch: each chunk's aggregated (label+1, frequency)
+// pairs, the number of pairs per chunk, and per-hub winners (zero between
+// steps).
 struct HubScratch {
-    unsigned long long* tbl = nullptr;
-    uint32_t* occ = nullptr;
-    uint32_t* occ_cnt = nullptr;
+    unsigned long long* pairs = nullptr;
+    uint32_t* chunk_cnt = nullptr;
     unsigned long long* best = nullptr;
+};
+
+// Hub-tier launch arguments (plan lists plus session scratch).
+struct HubArgs {
+    const int32_t* hubs;
+    const int2* chunks;       // (hub, chunk) per aggregation workgroup
+    const int2* parts;        // (hub, part) per counting workgroup
+    const int32_t* chunk0;    // first chunk index of each hub
+    const int64_t* base;      // first pair slot of each hub
+    int32_t chunk;            // edges per chunk = pair slots per chunk region
+    int32_t part_keys;        // expected distinct labels per counting workgroup
+    int32_t part_limit;       // distinct labels that trigger halving (0: table/2)
+    int32_t nhubs;
+    HubScratch hs;
+};
+
+// One job's share of a folded (multi-job) step: its step arguments and
+// tier work lists.  blocks[] holds the workgroups it needs in each launch,
+// in the order LAUNCH_* below.
+enum { kLaunchPacked = 0, kLaunchWave = 1, kLaunchBlock0 = 2,
+       kLaunchHubAgg = 2 + kMaxBlockClasses, kLaunchHubPart, kLaunchHubFinal, kLaunches };
+struct JobDesc {
+    StepArgs a;
+    const int2* groups;
+    const int32_t* wave;
+    const int32_t* block[kMaxBlockClasses];
+    HubArgs hub;
+    int32_t ngroups, nwave;
+    int32_t blocks[kLaunches];
 };
 
 enum class Method { Classic, Layered, Slpa, User };
@@ -98,5 +133,12 @@ int launch_fill_i32(int32_t* p, int32_t value, int64_t count, hipStream_t s);
 int launch_iota(int32_t* p, int64_t count, hipStream_t s);
 int launch_end_step(unsigned long long* acc, unsigned long long* ring, int32_t* step_ctr,
                     int32_t ring_size, hipStream_t s);
+
+// Folded step for nj jobs with layered/classic hooks (gamma 0 = classic).
+// jobs and cum are device arrays; grid (host) holds each launch's size.
+int launch_multi(bool weighted, const JobDesc* jobs, const int32_t* cum, int32_t nj,
+                 const int32_t* grid, const Tuning& t, const int32_t* class_cap, hipStream_t s);
+// Workgroups each folded launch needs for a job with plan p.
+void job_blocks(const Plan& p, int32_t* blocks);
 
 }  // namespace glp

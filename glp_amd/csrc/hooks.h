@@ -32,7 +32,7 @@ struct StepArgs {
     int32_t* Lw;             // labels written by this step
     const int32_t* Vr;       // layered: volume of each label before the step
     int32_t* Vw;             // layered: volume after the step (zeroed first)
-    unsigned long long* changes;  // accumulates the number of changed labels
+    unsigned long long* changes;  // kChangeSlots counters of changed labels (summed per step)
     int32_t score_mul;       // layered: q+p for gamma = p/q
     int32_t score_vmul;      // layered: p
     int32_t n;
@@ -62,7 +62,8 @@ struct ClassicHooks {
     __device__ static __forceinline__ uint32_t load_neighbor(const StepArgs& a, int32_t vid,
                                                              int32_t j, uint32_t* freq) {
         (void)vid;
-        *freq = W ? a.w[j] : 1u;
+        // a weighted launch may fold in unweighted jobs (w == nullptr)
+        *freq = W ? (a.w ? a.w[j] : 1u) : 1u;
         return (uint32_t)(G ? a.adj[j] : a.L[a.nbr[j]]) + 1u;
     }
     __device__ static __forceinline__ uint64_t label_score(const StepArgs& a, int32_t vid,
@@ -93,15 +94,17 @@ struct LayeredHooks {
     __device__ static __forceinline__ uint64_t label_score(const StepArgs& a, int32_t vid,
                                                            uint32_t key, uint32_t freq) {
         (void)vid;
-        // both terms are range-checked on the host to fit int32
-        int32_t s = a.score_mul * (int32_t)freq - a.score_vmul * a.Vr[key - 1];
+        // both terms are range-checked on the host to fit int32; with
+        // score_vmul = 0 (gamma = 0) this is classic LP and Vr may be null
+        int32_t s = a.score_mul * (int32_t)freq;
+        if (a.score_vmul) s -= a.score_vmul * a.Vr[key - 1];
         return pack_score(s, key);
     }
     __device__ static __forceinline__ void update_vertex(const StepArgs& a, int32_t vid,
                                                          int32_t label, uint64_t score) {
         (void)score;
         a.Lw[vid] = label;
-        atomicAdd(&a.Vw[label], 1);  // header/kernel.cuh:2626 gcounter_temp
+        if (a.Vw) atomicAdd(&a.Vw[label], 1);  // header/kernel.cuh:2626 gcounter_temp
     }
 };
 

@@ -57,17 +57,46 @@ __device__ __forceinline__ void lds_insert(uint32_t* K, uint32_t* C, uint32_t ma
     }
 }
 
+// Adds this wave's changed-label count to one of kChangeSlots counters
+// (chosen by workgroup), so no single address takes every wave's atomic.
+// Counts edges [b, e) of vertex v into an LDS table, STRIDE threads
+// cooperating (thread `tid`).  Each thread first issues the loads of K edges
+// (neighbour ids, then their labels) and only then inserts them, so K
+// gathers are in flight per thread instead of one.
+template <class H, bool W, bool G, int K, int STRIDE>
+__device__ __forceinline__ void count_edges(const StepArgs& a, int32_t v, int32_t b, int32_t e,
+                                            int tid, uint32_t* Kt, uint32_t* Ct, uint32_t mask) {
+    for (int32_t j0 = b + tid; j0 < e; j0 += STRIDE * K) {
+        uint32_t key[K], fr[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const int32_t j = j0 + i * STRIDE;
+            key[i] = 0u;
+            fr[i] = 0u;
+            if (j < e) key[i] = H::template load_neighbor<W, G>(a, v, j, &fr[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            if (key[i]) lds_insert(Kt, Ct, mask, key[i], fr[i]);
+    }
+}
+
 __device__ __forceinline__ void count_changes(const StepArgs& a, bool changed) {
     unsigned long long bal = __ballot(changed);
-    if (lane_id() == 0 && bal) atomicAdd(a.changes, (unsigned long long)__popcll(bal));
+    if (lane_id() == 0 && bal)
+        atomicAdd(&a.changes[blockIdx.x & (kChangeSlots - 1)], (unsigned long long)__popcll(bal));
 }
+
+// Each tier is a __device__ body taking its workgroup index `blk`; the
+// __global__ wrappers below run it for one job (blk = blockIdx.x) or for a
+// table of jobs folded into one launch (blk = index inside the owning job).
 
 // ---------------------------------------------------------------- packed ----
 template <class H, bool W, bool G>
-__global__ __launch_bounds__(256) void k_packed(StepArgs a, const int2* __restrict__ groups,
-                                                int32_t ngroups) {
+__device__ __forceinline__ void packed_body(const StepArgs& a, const int2* __restrict__ groups,
+                                            int32_t ngroups, uint32_t blk) {
     const int lane = lane_id();
-    const int32_t gid = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int32_t gid = (int32_t)((blk * 256u + threadIdx.x) >> 6);
     if (gid >= ngroups) return;  // wave-uniform
     const int2 gr = groups[gid];
     const int32_t v0 = gr.x, vc = gr.y;
@@ -124,12 +153,12 @@ __global__ __launch_bounds__(256) void k_packed(StepArgs a, const int2* __restri
 
 // ------------------------------------------------------------------ wave ----
 template <class H, bool W, bool G>
-__global__ __launch_bounds__(256) void k_wave(StepArgs a, const int32_t* __restrict__ verts,
-                                              int32_t nv) {
+__device__ __forceinline__ void wave_body(const StepArgs& a, const int32_t* __restrict__ verts,
+                                          int32_t nv, uint32_t blk) {
     __shared__ uint32_t s_key[4][kWaveCap];
     __shared__ uint32_t s_cnt[4][kWaveCap];
     const int lane = lane_id(), wv = threadIdx.x >> 6;
-    const int32_t idx = (int32_t)blockIdx.x * 4 + wv;
+    const int32_t idx = (int32_t)blk * 4 + wv;
     if (idx >= nv) return;  // wave-uniform
     const int32_t v = verts[idx];
     const int32_t b = a.off[v], e = a.off[v + 1];
@@ -142,11 +171,7 @@ __global__ __launch_bounds__(256) void k_wave(StepArgs a, const int32_t* __restr
         C[i] = 0u;
     }
     wave_lds_sync();
-    for (int32_t j = b + lane; j < e; j += 64) {
-        uint32_t fr;
-        const uint32_t key = H::template load_neighbor<W, G>(a, v, j, &fr);
-        lds_insert(K, C, mask, key, fr);
-    }
+    count_edges<H, W, G, 4, 64>(a, v, b, e, lane, K, C, mask);
     wave_lds_sync();
     uint64_t best = 0ull;
     for (uint32_t i = lane; i < cap; i += 64) {
@@ -179,13 +204,13 @@ __device__ __forceinline__ uint64_t block_max_u64(uint64_t x, unsigned long long
 }
 
 template <class H, bool W, bool G, int NT>
-__global__ __launch_bounds__(NT) void k_block(StepArgs a, const int32_t* __restrict__ verts,
-                                              uint32_t cap_max) {
+__device__ __forceinline__ void block_body(const StepArgs& a, const int32_t* __restrict__ verts,
+                                           uint32_t cap_max, uint32_t blk) {
     extern __shared__ uint32_t s_dyn[];
     __shared__ unsigned long long s_red[NT / 64];
     uint32_t* K = s_dyn;
     uint32_t* C = s_dyn + cap_max;
-    const int32_t v = verts[blockIdx.x];
+    const int32_t v = verts[blk];
     const int32_t b = a.off[v], e = a.off[v + 1];
     const uint32_t cap = min(max(next_pow2(2u * (uint32_t)(e - b)), 64u), cap_max);
     const uint32_t mask = cap - 1u;
@@ -194,11 +219,7 @@ __global__ __launch_bounds__(NT) void k_block(StepArgs a, const int32_t* __restr
         C[i] = 0u;
     }
     __syncthreads();
-    for (int32_t j = b + (int32_t)threadIdx.x; j < e; j += NT) {
-        uint32_t fr;
-        const uint32_t key = H::template load_neighbor<W, G>(a, v, j, &fr);
-        lds_insert(K, C, mask, key, fr);
-    }
+    count_edges<H, W, G, 8, NT>(a, v, b, e, threadIdx.x, K, C, mask);
     __syncthreads();
     uint64_t best = 0ull;
     for (uint32_t i = threadIdx.x; i < cap; i += NT) {
@@ -219,104 +240,309 @@ __global__ __launch_bounds__(NT) void k_block(StepArgs a, const int32_t* __restr
 }
 
 // ------------------------------------------------------------------- hub ----
-struct HubArgs {
-    const int32_t* hubs;
-    const int2* chunks;
-    const int64_t* tbl_off;
-    const uint32_t* cap;
-    const int64_t* occ_off;
-    int32_t chunk;
-    int32_t nhubs;
-    HubScratch hs;
-};
+// Hubs are counted in two passes and never touch an HBM hash table:
+//  k_hub_agg   one workgroup per chunk of `chunk` edges: gathers the labels,
+//              counts them in an LDS table and writes the chunk's distinct
+//              (label+1, frequency) pairs into the chunk's own pair region
+//  k_hub_part  one workgroup per (hub, part): owns the labels whose fmix32
+//              image falls in the part's share of the 32-bit range, streams
+//              the hub's pairs (L2-resident) and sums the matching ones in an
+//              LDS table.  The number of parts in use follows the pairs the
+//              chunks reported, so converged hubs need one part.  If a range
+//              holds more labels than the table, the workgroup counts the
+//              first half of the range, then the rest, halving again as
+//              needed; fmix32 is a bijection, so a range narrower than the
+//              table can never overflow and the loop always ends.
+//  k_hub_final one thread per hub: commits the winner.
+constexpr int kHubThreads = 256;
 
 template <class H, bool W, bool G>
-__global__ __launch_bounds__(256) void k_hub_count(StepArgs a, HubArgs h) {
-    __shared__ uint32_t K[kHubLdsCap];
-    __shared__ uint32_t C[kHubLdsCap];
-    const int2 hc = h.chunks[blockIdx.x];
+__device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
+    // LDS table of next_pow2(2*chunk) slots (dynamic, sized at launch)
+    extern __shared__ uint32_t s_dyn[];
+    __shared__ uint32_t s_npairs;
+    const uint32_t cap_max = next_pow2(2u * (uint32_t)h.chunk);
+    uint32_t* K = s_dyn;
+    uint32_t* C = s_dyn + cap_max;
+    const int2 hc = h.chunks[blk];
     const int32_t v = h.hubs[hc.x];
     const int32_t vb = a.off[v], ve = a.off[v + 1];
     const int32_t b = vb + hc.y * h.chunk;
     const int32_t e = min(b + h.chunk, ve);
-    const uint32_t cap = min(max(next_pow2(2u * (uint32_t)(e - b)), 64u), (uint32_t)kHubLdsCap);
+    const uint32_t cap = min(max(next_pow2(2u * (uint32_t)(e - b)), 64u), cap_max);
     const uint32_t mask = cap - 1u;
-    for (uint32_t i = threadIdx.x; i < cap; i += 256) {
+    for (uint32_t i = threadIdx.x; i < cap; i += kHubThreads) {
         K[i] = 0u;
         C[i] = 0u;
     }
+    if (threadIdx.x == 0) s_npairs = 0u;
     __syncthreads();
-    for (int32_t j = b + (int32_t)threadIdx.x; j < e; j += 256) {
-        uint32_t fr;
-        const uint32_t key = H::template load_neighbor<W, G>(a, v, j, &fr);
-        lds_insert(K, C, mask, key, fr);
-    }
+    count_edges<H, W, G, 8, kHubThreads>(a, v, b, e, threadIdx.x, K, C, mask);
     __syncthreads();
-    unsigned long long* T = h.hs.tbl + h.tbl_off[hc.x];
-    const uint32_t gmask = h.cap[hc.x] - 1u;
-    for (uint32_t i = threadIdx.x; i < cap; i += 256) {
+    unsigned long long* out = h.hs.pairs + h.base[hc.x] + (int64_t)hc.y * h.chunk;
+    for (uint32_t i = threadIdx.x; i < cap; i += kHubThreads) {
         const uint32_t k = K[i];
         if (!k) continue;
-        const uint32_t c = C[i];
-        uint32_t pos = fmix32(k) & gmask;
-        for (;;) {
-            unsigned long long cur =
-                __hip_atomic_load(&T[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (cur == 0ull) {
-                const unsigned long long prev =
-                    atomicCAS(&T[pos], 0ull, ((unsigned long long)k << 32) | c);
-                if (prev == 0ull) {
-                    const uint32_t slot = atomicAdd(&h.hs.occ_cnt[hc.x], 1u);
-                    h.hs.occ[h.occ_off[hc.x] + slot] = pos;
+        const uint32_t slot = atomicAdd(&s_npairs, 1u);
+        out[slot] = ((unsigned long long)k << 32) | C[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) h.hs.chunk_cnt[blk] = s_npairs;
+}
+
+// Sums the (key, frequency) pairs whose fmix32 image lies in [lo, hi) into the
+// LDS table.  Returns false (and leaves the table partial) when more than
+// `limit` distinct keys arrive; the table has room for limit + kHubThreads.
+__device__ __forceinline__ bool part_pass(const unsigned long long* pairs, const uint32_t* cnt,
+                                          int32_t nch, int32_t chunk, uint64_t lo, uint64_t hi,
+                                          uint32_t* K, uint32_t* C, uint32_t shift,
+                                          uint32_t limit, uint32_t* s_distinct,
+                                          uint32_t* s_over) {
+    const uint32_t mask = (1u << (32 - shift)) - 1u;
+    for (int32_t c = 0; c < nch; ++c) {
+        const uint32_t nc = cnt[c];
+        const unsigned long long* pc = pairs + (int64_t)c * chunk;
+        for (uint32_t i = threadIdx.x; i < nc; i += kHubThreads) {
+            if (lds_load(s_over)) return false;
+            const unsigned long long pr = pc[i];
+            const uint32_t key = (uint32_t)(pr >> 32);
+            const uint32_t hk = fmix32(key);
+            if (hk < lo || hk >= hi) continue;
+            const uint32_t f = (uint32_t)pr;
+            uint32_t s = (hk * 0x9e3779b1u) >> shift;
+            for (;;) {
+                const uint32_t k = lds_load(&K[s]);
+                if (k == key) {
+                    atomicAdd(&C[s], f);
                     break;
                 }
-                cur = prev;
+                if (k == 0u) {
+                    const uint32_t prev = atomicCAS(&K[s], 0u, key);
+                    if (prev == 0u) {
+                        if (atomicAdd(s_distinct, 1u) + 1u > limit) atomicExch(s_over, 1u);
+                        atomicAdd(&C[s], f);
+                        break;
+                    }
+                    if (prev == key) {
+                        atomicAdd(&C[s], f);
+                        break;
+                    }
+                }
+                s = (s + 1u) & mask;
             }
-            if ((uint32_t)(cur >> 32) == k) {
-                atomicAdd(&T[pos], (unsigned long long)c);
-                break;
-            }
-            pos = (pos + 1u) & gmask;
         }
     }
+    return true;
 }
 
 template <class H>
-__global__ __launch_bounds__(256) void k_hub_reduce(StepArgs a, HubArgs h) {
-    __shared__ unsigned long long s_red[4];
-    const int2 hc = h.chunks[blockIdx.x];
-    const int32_t v = h.hubs[hc.x];
-    const uint32_t n_occ = h.hs.occ_cnt[hc.x];
-    const uint32_t lo = (uint32_t)hc.y * (uint32_t)h.chunk;
-    const uint32_t hi = min(lo + (uint32_t)h.chunk, n_occ);
-    unsigned long long* T = h.hs.tbl + h.tbl_off[hc.x];
-    const uint32_t* occ = h.hs.occ + h.occ_off[hc.x];
+__device__ __forceinline__ void hub_part_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
+    extern __shared__ uint32_t s_dyn[];
+    __shared__ unsigned long long s_red[kHubThreads / 64];
+    __shared__ uint32_t s_sum[kHubThreads / 64];
+    __shared__ uint32_t s_distinct, s_over;
+    const uint32_t cap = next_pow2(4u * (uint32_t)h.part_keys);
+    const uint32_t shift = 32u - (31u - __builtin_clz(cap));
+    uint32_t* K = s_dyn;
+    uint32_t* C = s_dyn + cap;
+    const int2 hp = h.parts[blk];
+    const int32_t v = h.hubs[hp.x];
+    const int32_t d = a.off[v + 1] - a.off[v];
+    const int32_t nch = (d + h.chunk - 1) / h.chunk;
+    const uint32_t* cnt = h.hs.chunk_cnt + h.chunk0[hp.x];
+    const unsigned long long* pairs = h.hs.pairs + h.base[hp.x];
+    // parts in use: enough for the distinct pairs the chunks reported
+    uint32_t tot = 0u;
+    for (int32_t c = threadIdx.x; c < nch; c += kHubThreads) tot += cnt[c];
+    tot = wave_sum_u32(tot);
+    if (lane_id() == 0) s_sum[threadIdx.x >> 6] = tot;
+    __syncthreads();
+    tot = 0u;
+    for (int i = 0; i < kHubThreads / 64; ++i) tot += s_sum[i];
+    const uint32_t nparts = max(1u, (tot + (uint32_t)h.part_keys - 1u) / (uint32_t)h.part_keys);
+    if ((uint32_t)hp.y >= nparts) return;  // block-uniform
+    const uint64_t span = ((1ull << 32) + nparts - 1) / nparts;
+    const uint64_t end = min((uint64_t)(hp.y + 1) * span, 1ull << 32);
+    uint64_t lo = (uint64_t)hp.y * span, hi = end;
+    const uint32_t limit = h.part_limit > 0 ? (uint32_t)h.part_limit : cap / 2u;
     uint64_t best = 0ull;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += 256) {
-        const uint32_t pos = occ[i];
-        const unsigned long long s = T[pos];
-        T[pos] = 0ull;
-        best = max_u64(best, H::label_score(a, v, (uint32_t)(s >> 32), (uint32_t)s));
+    while (lo < end) {
+        for (uint32_t i = threadIdx.x; i < cap; i += kHubThreads) {
+            K[i] = 0u;
+            C[i] = 0u;
+        }
+        if (threadIdx.x == 0) {
+            s_distinct = 0u;
+            s_over = 0u;
+        }
+        __syncthreads();
+        part_pass(pairs, cnt, nch, h.chunk, lo, hi, K, C, shift, limit, &s_distinct, &s_over);
+        __syncthreads();
+        const bool over = s_over != 0u;
+        __syncthreads();
+        if (over) {
+            // retry the lower half of the range (a range no wider than `limit`
+            // cannot overflow, so the width stays >= 1)
+            hi = lo + max((hi - lo) / 2, (uint64_t)1);
+            continue;
+        }
+        for (uint32_t i = threadIdx.x; i < cap; i += kHubThreads) {
+            const uint32_t k = K[i];
+            if (k) best = max_u64(best, H::label_score(a, v, k, C[i]));
+        }
+        __syncthreads();
+        lo = hi;
+        hi = end;
     }
-    best = block_max_u64<256>(best, s_red);
-    if (threadIdx.x == 0 && best) atomicMax(&h.hs.best[hc.x], (unsigned long long)best);
+    best = block_max_u64<kHubThreads>(best, s_red);
+    if (threadIdx.x == 0 && best) atomicMax(&h.hs.best[hp.x], (unsigned long long)best);
 }
 
 template <class H>
-__global__ __launch_bounds__(256) void k_hub_final(StepArgs a, HubArgs h) {
-    const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+__device__ __forceinline__ void hub_final_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
+    const int32_t i = (int32_t)(blk * 256u + threadIdx.x);
     bool changed = false;
     if (i < h.nhubs) {
         const int32_t v = h.hubs[i];
         const unsigned long long best = h.hs.best[i];
         h.hs.best[i] = 0ull;
-        h.hs.occ_cnt[i] = 0u;
         const int32_t old = a.keep[v];
         const int32_t nl = packed_label(best);
         H::update_vertex(a, v, nl, best);
         changed = nl != old;
     }
     count_changes(a, changed);
+}
+
+// ------------------------------------------------------------- wrappers ----
+// Single job: workgroup index = blockIdx.x.
+template <class H, bool W, bool G>
+__global__ __launch_bounds__(256) void k_packed(StepArgs a, const int2* __restrict__ groups,
+                                                int32_t ngroups) {
+    packed_body<H, W, G>(a, groups, ngroups, blockIdx.x);
+}
+template <class H, bool W, bool G>
+__global__ __launch_bounds__(256) void k_wave(StepArgs a, const int32_t* __restrict__ verts,
+                                              int32_t nv) {
+    wave_body<H, W, G>(a, verts, nv, blockIdx.x);
+}
+template <class H, bool W, bool G, int NT>
+__global__ __launch_bounds__(NT) void k_block(StepArgs a, const int32_t* __restrict__ verts,
+                                              uint32_t cap_max) {
+    block_body<H, W, G, NT>(a, verts, cap_max, blockIdx.x);
+}
+template <class H, bool W, bool G>
+__global__ __launch_bounds__(kHubThreads) void k_hub_agg(StepArgs a, HubArgs h) {
+    hub_agg_body<H, W, G>(a, h, blockIdx.x);
+}
+template <class H>
+__global__ __launch_bounds__(kHubThreads) void k_hub_part(StepArgs a, HubArgs h) {
+    hub_part_body<H>(a, h, blockIdx.x);
+}
+template <class H>
+__global__ __launch_bounds__(256) void k_hub_final(StepArgs a, HubArgs h) {
+    hub_final_body<H>(a, h, blockIdx.x);
+}
+
+// Folded jobs: cum[j] is the first workgroup of job j in this launch
+// (cum[njobs] = grid size); a workgroup finds its job by binary search.
+__device__ __forceinline__ int32_t job_of(const int32_t* __restrict__ cum, int32_t njobs,
+                                          uint32_t b) {
+    int32_t lo = 0, hi = njobs - 1;
+    while (lo < hi) {  // largest j with cum[j] <= b
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if ((uint32_t)cum[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <class H, bool W, bool G, int L, int NT>
+__global__ __launch_bounds__(NT) void k_multi(const JobDesc* __restrict__ jobs,
+                                              const int32_t* __restrict__ cum, int32_t njobs,
+                                              uint32_t cap_max) {
+    const int32_t j = job_of(cum, njobs, blockIdx.x);
+    const uint32_t blk = blockIdx.x - (uint32_t)cum[j];
+    const JobDesc& jd = jobs[j];
+    if constexpr (L == kLaunchPacked) {
+        packed_body<H, W, G>(jd.a, jd.groups, jd.ngroups, blk);
+    } else if constexpr (L == kLaunchWave) {
+        wave_body<H, W, G>(jd.a, jd.wave, jd.nwave, blk);
+    } else if constexpr (L >= kLaunchBlock0 && L < kLaunchBlock0 + kMaxBlockClasses) {
+        block_body<H, W, G, NT>(jd.a, jd.block[L - kLaunchBlock0], cap_max, blk);
+    } else if constexpr (L == kLaunchHubAgg) {
+        hub_agg_body<H, W, G>(jd.a, jd.hub, blk);
+    } else if constexpr (L == kLaunchHubPart) {
+        hub_part_body<H>(jd.a, jd.hub, blk);
+    } else {
+        hub_final_body<H>(jd.a, jd.hub, blk);
+    }
+}
+
+// Folded launches of one step for `nj` jobs (the serving path).  cum holds
+// kLaunches rows of nj+1 prefix sums; grid[L] = cum row L's last entry.
+template <class H, bool W, int C>
+static void launch_multi_block(const JobDesc* jobs, const int32_t* cum, int32_t nj,
+                               const int32_t* grid, const Tuning& t, uint32_t cap, hipStream_t s) {
+    constexpr int L = kLaunchBlock0 + C;
+    if (!grid[L]) return;
+    const size_t lds = (size_t)cap * 2 * sizeof(uint32_t);
+    const int32_t* row = cum + (size_t)L * (nj + 1);
+    if (t.block_threads == 1024)
+        hipLaunchKernelGGL((k_multi<H, W, false, L, 1024>), dim3(grid[L]), dim3(1024), lds, s, jobs, row, nj, cap);
+    else if (t.block_threads == 512)
+        hipLaunchKernelGGL((k_multi<H, W, false, L, 512>), dim3(grid[L]), dim3(512), lds, s, jobs, row, nj, cap);
+    else
+        hipLaunchKernelGGL((k_multi<H, W, false, L, 256>), dim3(grid[L]), dim3(256), lds, s, jobs, row, nj, cap);
+}
+
+template <class H, bool W>
+static int launch_multi_t(const JobDesc* jobs, const int32_t* cum, int32_t nj, const int32_t* grid,
+                          const Tuning& t, const int32_t* class_cap, hipStream_t s) {
+    auto row = [&](int L) { return cum + (size_t)L * (nj + 1); };
+    if (grid[kLaunchPacked])
+        hipLaunchKernelGGL((k_multi<H, W, false, kLaunchPacked, 256>), dim3(grid[kLaunchPacked]),
+                           dim3(256), 0, s, jobs, row(kLaunchPacked), nj, 0u);
+    if (grid[kLaunchWave])
+        hipLaunchKernelGGL((k_multi<H, W, false, kLaunchWave, 256>), dim3(grid[kLaunchWave]),
+                           dim3(256), 0, s, jobs, row(kLaunchWave), nj, 0u);
+    launch_multi_block<H, W, 0>(jobs, cum, nj, grid, t, (uint32_t)class_cap[0], s);
+    launch_multi_block<H, W, 1>(jobs, cum, nj, grid, t, (uint32_t)class_cap[1], s);
+    launch_multi_block<H, W, 2>(jobs, cum, nj, grid, t, (uint32_t)class_cap[2], s);
+    launch_multi_block<H, W, 3>(jobs, cum, nj, grid, t, (uint32_t)class_cap[3], s);
+    launch_multi_block<H, W, 4>(jobs, cum, nj, grid, t, (uint32_t)class_cap[4], s);
+    if (grid[kLaunchHubAgg]) {
+        const size_t lds = (size_t)next_pow2(2u * (uint32_t)t.hub_chunk) * 2 * sizeof(uint32_t);
+        hipLaunchKernelGGL((k_multi<H, W, false, kLaunchHubAgg, kHubThreads>), dim3(grid[kLaunchHubAgg]),
+                           dim3(kHubThreads), lds, s, jobs, row(kLaunchHubAgg), nj, 0u);
+    }
+    if (grid[kLaunchHubPart]) {
+        const size_t lds = (size_t)next_pow2(4u * (uint32_t)t.part_keys) * 2 * sizeof(uint32_t);
+        hipLaunchKernelGGL((k_multi<H, W, false, kLaunchHubPart, kHubThreads>), dim3(grid[kLaunchHubPart]),
+                           dim3(kHubThreads), lds, s, jobs, row(kLaunchHubPart), nj, 0u);
+    }
+    if (grid[kLaunchHubFinal])
+        hipLaunchKernelGGL((k_multi<H, W, false, kLaunchHubFinal, 256>), dim3(grid[kLaunchHubFinal]),
+                           dim3(256), 0, s, jobs, row(kLaunchHubFinal), nj, 0u);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "folded launch: %s", hipGetErrorString(e));
+}
+
+int launch_multi(bool weighted, const JobDesc* jobs, const int32_t* cum, int32_t nj,
+                 const int32_t* grid, const Tuning& t, const int32_t* class_cap, hipStream_t s) {
+    if (weighted) return launch_multi_t<LayeredHooks, true>(jobs, cum, nj, grid, t, class_cap, s);
+    return launch_multi_t<LayeredHooks, false>(jobs, cum, nj, grid, t, class_cap, s);
+}
+
+// Workgroups each launch of a folded step needs for one job.
+void job_blocks(const Plan& p, int32_t* blocks) {
+    for (int L = 0; L < kLaunches; ++L) blocks[L] = 0;
+    blocks[kLaunchPacked] = (int32_t)((p.groups.size() + 3) / 4);
+    blocks[kLaunchWave] = (int32_t)((p.wave.size() + 3) / 4);
+    for (int c = 0; c < p.nclasses; ++c) blocks[kLaunchBlock0 + c] = (int32_t)p.block[c].size();
+    blocks[kLaunchHubAgg] = (int32_t)p.hub_chunks.size();
+    blocks[kLaunchHubPart] = (int32_t)p.hub_parts.size();
+    blocks[kLaunchHubFinal] = (int32_t)((p.hubs.size() + 255) / 256);
 }
 
 // ------------------------------------------------------------ auxiliaries ----
@@ -359,13 +585,24 @@ __global__ void k_iota(int32_t* p, int64_t count) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
         p[i] = (int32_t)i;
 }
-// Moves the step's change count into the ring and advances the step counter.
-__global__ void k_end_step(unsigned long long* acc, unsigned long long* ring, int32_t* ctr,
-                           int32_t ring_size) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+// Sums and clears the step's change counters into the ring and advances
+// the step counter.  One workgroup of kChangeSlots threads.
+__global__ __launch_bounds__(kChangeSlots) void k_end_step(unsigned long long* acc,
+                                                           unsigned long long* ring, int32_t* ctr,
+                                                           int32_t ring_size) {
+    __shared__ unsigned long long part[kChangeSlots / 64];
+    unsigned long long x = acc[threadIdx.x];
+    acc[threadIdx.x] = 0ull;
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x += shfl_u64(x, lane ^ d);
+    if (lane == 0) part[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long sum = 0ull;
+        for (int i = 0; i < kChangeSlots / 64; ++i) sum += part[i];
         const int32_t t = *ctr;
-        ring[t % ring_size] = *acc;
-        *acc = 0ull;
+        ring[t % ring_size] = sum;
         *ctr = t + 1;
     }
 }
@@ -420,17 +657,22 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
         HubArgs h;
         h.hubs = g.d_hubs;
         h.chunks = g.d_hub_chunks;
-        h.tbl_off = g.d_hub_tbl_off;
-        h.cap = g.d_hub_cap;
-        h.occ_off = g.d_hub_occ_off;
+        h.parts = g.d_hub_parts;
+        h.chunk0 = g.d_hub_chunk0;
+        h.base = g.d_hub_base;
         h.chunk = p.tune.hub_chunk;
+        h.part_keys = p.tune.part_keys;
+        h.part_limit = p.tune.part_limit;
         h.nhubs = (int32_t)p.hubs.size();
         h.hs = hs;
         const unsigned nc = (unsigned)p.hub_chunks.size();
-        hipLaunchKernelGGL((k_hub_count<H, W, G>), dim3(nc), dim3(256), 0, s, a, h);
+        const unsigned np = (unsigned)p.hub_parts.size();
+        const size_t lds_agg = (size_t)next_pow2(2u * (uint32_t)h.chunk) * 2 * sizeof(uint32_t);
+        const size_t lds_part = (size_t)next_pow2(4u * (uint32_t)h.part_keys) * 2 * sizeof(uint32_t);
+        hipLaunchKernelGGL((k_hub_agg<H, W, G>), dim3(nc), dim3(kHubThreads), lds_agg, s, a, h);
         mark(GLP_TIER_HUB, 1);
         mark(GLP_TIER_AUX, 0);
-        hipLaunchKernelGGL((k_hub_reduce<H>), dim3(nc), dim3(256), 0, s, a, h);
+        hipLaunchKernelGGL((k_hub_part<H>), dim3(np), dim3(kHubThreads), lds_part, s, a, h);
         hipLaunchKernelGGL((k_hub_final<H>), dim3(grid_for(h.nhubs, 256)), dim3(256), 0, s, a, h);
         mark(GLP_TIER_AUX, 1);
     } else {
@@ -466,14 +708,35 @@ void init_kernels() {
                               hipFuncAttributeMaxDynamicSharedMemorySize, lim);                 \
     (void)hipFuncSetAttribute((const void*)k_block<HK, W, G, 1024>,                             \
                               hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+#define GLP_SETLDS_HUB(HK, W, G)                                                                \
+    (void)hipFuncSetAttribute((const void*)k_hub_agg<HK, W, G>,                                 \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim);
 #define GLP_SETLDS4(HK) GLP_SETLDS(HK, false, false) GLP_SETLDS(HK, false, true) \
-                        GLP_SETLDS(HK, true, false) GLP_SETLDS(HK, true, true)
+                        GLP_SETLDS(HK, true, false) GLP_SETLDS(HK, true, true)   \
+                        GLP_SETLDS_HUB(HK, false, false) GLP_SETLDS_HUB(HK, false, true) \
+                        GLP_SETLDS_HUB(HK, true, false) GLP_SETLDS_HUB(HK, true, true)   \
+    (void)hipFuncSetAttribute((const void*)k_hub_part<HK>,                                      \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     GLP_SETLDS4(ClassicHooks)
     GLP_SETLDS4(LayeredHooks)
     GLP_SETLDS4(SlpaHooks)
     GLP_SETLDS4(UserHooks)
 #undef GLP_SETLDS4
+#undef GLP_SETLDS_HUB
 #undef GLP_SETLDS
+#define GLP_SETLDS_MULTI(W, L, NT)                                                              \
+    (void)hipFuncSetAttribute((const void*)k_multi<LayeredHooks, W, false, L, NT>,              \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+#define GLP_SETLDS_MULTI_C(W, C) GLP_SETLDS_MULTI(W, kLaunchBlock0 + C, 256)                    \
+    GLP_SETLDS_MULTI(W, kLaunchBlock0 + C, 512) GLP_SETLDS_MULTI(W, kLaunchBlock0 + C, 1024)
+#define GLP_SETLDS_MULTI_W(W) GLP_SETLDS_MULTI_C(W, 0) GLP_SETLDS_MULTI_C(W, 1)                 \
+    GLP_SETLDS_MULTI_C(W, 2) GLP_SETLDS_MULTI_C(W, 3) GLP_SETLDS_MULTI_C(W, 4)                   \
+    GLP_SETLDS_MULTI(W, kLaunchHubAgg, kHubThreads) GLP_SETLDS_MULTI(W, kLaunchHubPart, kHubThreads)
+    GLP_SETLDS_MULTI_W(false)
+    GLP_SETLDS_MULTI_W(true)
+#undef GLP_SETLDS_MULTI_W
+#undef GLP_SETLDS_MULTI_C
+#undef GLP_SETLDS_MULTI
     (void)hipGetLastError();
 }
 
@@ -530,7 +793,7 @@ int launch_iota(int32_t* p, int64_t count, hipStream_t s) {
 
 int launch_end_step(unsigned long long* acc, unsigned long long* ring, int32_t* step_ctr,
                     int32_t ring_size, hipStream_t s) {
-    hipLaunchKernelGGL(k_end_step, dim3(1), dim3(64), 0, s, acc, ring, step_ctr, ring_size);
+    hipLaunchKernelGGL(k_end_step, dim3(1), dim3(kChangeSlots), 0, s, acc, ring, step_ctr, ring_size);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "end_step launch: %s", hipGetErrorString(e));
 }

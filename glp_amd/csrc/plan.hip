@@ -19,6 +19,37 @@
 
 namespace glp {
 
+namespace {
+struct TuningField {
+    const char* name;
+    int32_t Tuning::*field;
+};
+const TuningField kFields[] = {
+    {"pack_max_deg", &Tuning::pack_max_deg}, {"wave_max_deg", &Tuning::wave_max_deg},
+    {"block_max_deg", &Tuning::block_max_deg}, {"hub_chunk", &Tuning::hub_chunk},
+    {"part_keys", &Tuning::part_keys},       {"part_limit", &Tuning::part_limit},
+    {"block_threads", &Tuning::block_threads},
+};
+}  // namespace
+
+int tuning_set(Tuning& t, const char* key, int32_t value) {
+    for (const auto& f : kFields)
+        if (key && std::string(key) == f.name) {
+            t.*(f.field) = value;
+            return GLP_OK;
+        }
+    return set_error(GLP_E_ARG, "unknown tuning key '%s'", key ? key : "(null)");
+}
+
+int tuning_get(const Tuning& t, const char* key, int32_t* value) {
+    for (const auto& f : kFields)
+        if (key && std::string(key) == f.name) {
+            *value = t.*(f.field);
+            return GLP_OK;
+        }
+    return set_error(GLP_E_ARG, "unknown tuning key '%s'", key ? key : "(null)");
+}
+
 Tuning& default_tuning() {
     static Tuning t;
     static bool loaded = false;
@@ -33,13 +64,7 @@ Tuning& default_tuning() {
                 std::istringstream ss(line);
                 std::string k;
                 int v;
-                if (std::getline(ss, k, '=') && (ss >> v)) {
-                    if (k == "pack_max_deg") t.pack_max_deg = v;
-                    else if (k == "wave_max_deg") t.wave_max_deg = v;
-                    else if (k == "block_max_deg") t.block_max_deg = v;
-                    else if (k == "hub_chunk") t.hub_chunk = v;
-                    else if (k == "block_threads") t.block_threads = v;
-                }
+                if (std::getline(ss, k, '=') && (ss >> v)) (void)tuning_set(t, k.c_str(), v);
             }
         }
     }
@@ -53,8 +78,13 @@ static int check_tuning(const Tuning& t) {
     // the largest LDS table (2*deg slots of 8 bytes) must fit a CU's 160 KB
     if (t.block_max_deg < t.wave_max_deg || t.block_max_deg > 8192)
         return set_error(GLP_E_ARG, "block_max_deg must be in [wave_max_deg, 8192]");
-    if (t.hub_chunk < 256 || t.hub_chunk > kHubLdsCap / 2 || (t.hub_chunk & 255))
-        return set_error(GLP_E_ARG, "hub_chunk must be a multiple of 256 in [256, %d]", kHubLdsCap / 2);
+    // LDS tables: next_pow2(2*hub_chunk) and next_pow2(4*part_keys) slots of 8 bytes
+    if (t.hub_chunk < 256 || t.hub_chunk > 8192 || (t.hub_chunk & 255))
+        return set_error(GLP_E_ARG, "hub_chunk must be a multiple of 256 in [256, 8192]");
+    if (t.part_keys < 256 || t.part_keys > 4096)
+        return set_error(GLP_E_ARG, "part_keys must be in [256, 4096]");
+    if (t.part_limit < 0 || t.part_limit > 2 * t.part_keys)
+        return set_error(GLP_E_ARG, "part_limit must be in [0, 2*part_keys]");
     if (t.block_threads != 256 && t.block_threads != 512 && t.block_threads != 1024)
         return set_error(GLP_E_ARG, "block_threads must be 256, 512 or 1024");
     return GLP_OK;
@@ -116,15 +146,15 @@ int build_plan(const HostCSR& g, const Tuning& t, Plan& p) {
             const int32_t h = (int32_t)p.hubs.size();
             p.hubs.push_back(v);
             const int32_t nch = (d + t.hub_chunk - 1) / t.hub_chunk;
+            p.hub_chunk0.push_back((int32_t)p.hub_chunks.size());
             for (int32_t c = 0; c < nch; ++c) p.hub_chunks.push_back(make_int2(h, c));
             hub_chunks += nch;
-            const uint32_t distinct = (uint32_t)std::min<int64_t>(d, n);
-            const uint32_t cap = std::max(dev::next_pow2(2u * distinct), 64u);
-            p.hub_tbl_off.push_back(p.tbl_slots);
-            p.hub_cap.push_back(cap);
-            p.hub_occ_off.push_back(p.occ_slots);
-            p.tbl_slots += cap;
-            p.occ_slots += distinct;
+            p.hub_base.push_back(p.pair_slots);
+            p.pair_slots += (int64_t)nch * t.hub_chunk;
+            // enough counting workgroups for d distinct labels; the kernel uses
+            // fewer when the chunks report fewer
+            const int32_t nparts = (d + t.part_keys - 1) / t.part_keys;
+            for (int32_t q = 0; q < nparts; ++q) p.hub_parts.push_back(make_int2(h, q));
             p.edges[3] += d;
             p.verts[3]++;
         }
@@ -135,3 +165,14 @@ int build_plan(const HostCSR& g, const Tuning& t, Plan& p) {
 }
 
 }  // namespace glp
+
+extern "C" int glp_tuning_set(const char* key, int32_t value) {
+    glp::clear_error();
+    return glp::tuning_set(glp::default_tuning(), key, value);
+}
+
+extern "C" int glp_tuning_get(const char* key, int32_t* value) {
+    glp::clear_error();
+    if (!value) return glp::set_error(GLP_E_ARG, "null value");
+    return glp::tuning_get(glp::default_tuning(), key, value);
+}

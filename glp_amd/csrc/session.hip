@@ -30,9 +30,9 @@ DeviceGraph::~DeviceGraph() {
     for (int c = 0; c < kMaxBlockClasses; ++c) (void)hipFree(d_block[c]);
     (void)hipFree(d_hubs);
     (void)hipFree(d_hub_chunks);
-    (void)hipFree(d_hub_tbl_off);
-    (void)hipFree(d_hub_cap);
-    (void)hipFree(d_hub_occ_off);
+    (void)hipFree(d_hub_chunk0);
+    (void)hipFree(d_hub_base);
+    (void)hipFree(d_hub_parts);
 }
 
 template <class T>
@@ -56,12 +56,23 @@ size_t device_graph_bytes(const HostCSR& g, const Plan& p) {
 
 int device_graph_create(const HostCSR& g, const Tuning& t, hipStream_t s, DeviceGraph** out) {
     *out = nullptr;
+    double t0 = now_ms();
+    Plan plan;
+    int rc = build_plan(g, t, plan);
+    if (rc != GLP_OK) return rc;
+    double t1 = now_ms();
+    rc = device_graph_upload(g, std::move(plan), s, out);
+    if (rc == GLP_OK) (*out)->plan_ms = t1 - t0;
+    return rc;
+}
+
+int device_graph_upload(const HostCSR& g, Plan&& plan, hipStream_t s, DeviceGraph** out) {
+    *out = nullptr;
     auto dg = std::make_unique<DeviceGraph>();
     dg->n = g.n;
     dg->m = g.m;
-    double t0 = now_ms();
-    int rc = build_plan(g, t, dg->plan);
-    if (rc != GLP_OK) return rc;
+    dg->plan = std::move(plan);
+    int rc;
     double t1 = now_ms();
     size_t bytes = 0;
     const Plan& p = dg->plan;
@@ -74,12 +85,11 @@ int device_graph_create(const HostCSR& g, const Tuning& t, hipStream_t s, Device
         if ((rc = upload(&dg->d_block[c], p.block[c].data(), p.block[c].size(), s, bytes))) return rc;
     if ((rc = upload(&dg->d_hubs, p.hubs.data(), p.hubs.size(), s, bytes))) return rc;
     if ((rc = upload(&dg->d_hub_chunks, p.hub_chunks.data(), p.hub_chunks.size(), s, bytes))) return rc;
-    if ((rc = upload(&dg->d_hub_tbl_off, p.hub_tbl_off.data(), p.hub_tbl_off.size(), s, bytes))) return rc;
-    if ((rc = upload(&dg->d_hub_cap, p.hub_cap.data(), p.hub_cap.size(), s, bytes))) return rc;
-    if ((rc = upload(&dg->d_hub_occ_off, p.hub_occ_off.data(), p.hub_occ_off.size(), s, bytes))) return rc;
+    if ((rc = upload(&dg->d_hub_chunk0, p.hub_chunk0.data(), p.hub_chunk0.size(), s, bytes))) return rc;
+    if ((rc = upload(&dg->d_hub_base, p.hub_base.data(), p.hub_base.size(), s, bytes))) return rc;
+    if ((rc = upload(&dg->d_hub_parts, p.hub_parts.data(), p.hub_parts.size(), s, bytes))) return rc;
     GLP_HIP(hipStreamSynchronize(s));
     double t2 = now_ms();
-    dg->plan_ms = t1 - t0;
     dg->upload_ms = t2 - t1;
     dg->bytes = bytes;
     *out = dg.release();
@@ -159,9 +169,8 @@ glp_session::~glp_session() {
     (void)hipFree(shown);
     (void)hipFree(adj);
     (void)hipFree(mem);
-    (void)hipFree(hs.tbl);
-    (void)hipFree(hs.occ);
-    (void)hipFree(hs.occ_cnt);
+    (void)hipFree(hs.pairs);
+    (void)hipFree(hs.chunk_cnt);
     (void)hipFree(hs.best);
     (void)hipFree(d_acc);
     (void)hipFree(d_ring);
@@ -184,8 +193,8 @@ size_t session_bytes(const HostCSR& g, const Plan& p, const glp_params& prm) {
     if (prm.method == GLP_SLPA) b += n * 4 + (size_t)(prm.slpa_T + 1) * n * 4;
     if (prm.method == GLP_USER) b += n * 4;
     if (prm.gather) b += (size_t)g.m * 4;
-    b += (size_t)p.tbl_slots * 8 + (size_t)p.occ_slots * 4 + p.hubs.size() * 12;
-    b += 8 + 8 * kChangeRing + 4;
+    b += (size_t)p.pair_slots * 8 + p.hub_chunks.size() * 4 + p.hubs.size() * 8;
+    b += 8 * kChangeSlots + 8 * kChangeRing + 4;
     return b;
 }
 
@@ -221,16 +230,13 @@ int session_create(glp_graph* g, const glp_params* prm, glp_session** out) {
     if (prm->method == GLP_SLPA)
         GLP_HIP(hipMalloc(&s->mem, (size_t)(prm->slpa_T + 1) * nn * 4));
     if (prm->gather) GLP_HIP(hipMalloc(&s->adj, std::max<size_t>((size_t)g->csr.m, 1) * 4));
-    if (p.tbl_slots) {
-        GLP_HIP(hipMalloc(&s->hs.tbl, (size_t)p.tbl_slots * 8));
-        GLP_HIP(hipMemsetAsync(s->hs.tbl, 0, (size_t)p.tbl_slots * 8, s->stream));
-        GLP_HIP(hipMalloc(&s->hs.occ, (size_t)p.occ_slots * 4));
-        GLP_HIP(hipMalloc(&s->hs.occ_cnt, p.hubs.size() * 4));
-        GLP_HIP(hipMemsetAsync(s->hs.occ_cnt, 0, p.hubs.size() * 4, s->stream));
+    if (!p.hubs.empty()) {
+        GLP_HIP(hipMalloc(&s->hs.pairs, (size_t)p.pair_slots * 8));
+        GLP_HIP(hipMalloc(&s->hs.chunk_cnt, p.hub_chunks.size() * 4));
         GLP_HIP(hipMalloc(&s->hs.best, p.hubs.size() * 8));
         GLP_HIP(hipMemsetAsync(s->hs.best, 0, p.hubs.size() * 8, s->stream));
     }
-    GLP_HIP(hipMalloc(&s->d_acc, 8));
+    GLP_HIP(hipMalloc(&s->d_acc, 8 * kChangeSlots));
     GLP_HIP(hipMalloc(&s->d_ring, 8 * kChangeRing));
     GLP_HIP(hipMalloc(&s->d_ctr, 4));
     GLP_HIP(hipEventCreate(&s->ev_start));
@@ -266,7 +272,7 @@ int session_reset(glp_session* s) {
         if ((rc = launch_fill_i32(s->V[1], 0, n, s->stream))) return rc;
     }
     if (s->mem && (rc = launch_iota(s->mem, n, s->stream))) return rc;  // M_v[0] = v
-    GLP_HIP(hipMemsetAsync(s->d_acc, 0, 8, s->stream));
+    GLP_HIP(hipMemsetAsync(s->d_acc, 0, 8 * kChangeSlots, s->stream));
     GLP_HIP(hipMemsetAsync(s->d_ring, 0, 8 * kChangeRing, s->stream));
     GLP_HIP(hipMemsetAsync(s->d_ctr, 0, 4, s->stream));
     GLP_HIP(hipStreamSynchronize(s->stream));

@@ -92,6 +92,16 @@ const char* glp_version(void);
 void glp_params_default(glp_params* p);           /* classic, unweighted, gamma 0  */
 int glp_device_count(int32_t* count);
 
+/* Process-wide tier thresholds and launch shapes applied to graphs whose
+ * device copy is created afterwards (the autotuner writes these; the
+ * GLP_TUNING environment variable names a key=value file read at first use).
+ * Replaces the compile-time constants of prop/classicLprop.cuh:178-198,
+ * 235, 302, 311-316, 381 (tier thresholds, block sizes, LDS table sizes).
+ * Keys: pack_max_deg, wave_max_deg, block_max_deg, hub_chunk, part_keys,
+ * part_limit, block_threads.                                                 */
+int glp_tuning_set(const char* key, int32_t value);
+int glp_tuning_get(const char* key, int32_t* value);
+
 /* ---- graphs ----------------------------------------------------------- */
 /* Replaces header/graph.h:69-147 (CSRGraph ctor) with :227-258
  * (load_edgelist_bin): headerless little-endian int32 (u,v) pairs, trailing

@@ -1,0 +1,214 @@
+"""GPU parity of the non-classic methods (layered, SLPA, count-min, frequent,
+buffer, user hooks) and of the batched and served forms against solo runs."""
+import numpy as np
+import pytest
+
+import glp_amd as G
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_steps_equal(got, want, what=""):
+    for t in range(len(want)):
+        bad = np.flatnonzero(got[t] != want[t])
+        assert len(bad) == 0, "%s step %d: %d differ, first v=%d gpu=%d oracle=%d" % (
+            what, t + 1, len(bad), bad[0], got[t][bad[0]], want[t][bad[0]])
+
+
+@pytest.fixture(scope="module")
+def rmat14():
+    g = G.Graph.rmat(14, 16, seed=31)
+    return g, g.csr()
+
+
+@pytest.fixture(scope="module")
+def rmat13w():
+    g = G.Graph.rmat(13, 16, seed=32, wmax=8)
+    return g, g.csr()
+
+
+# ------------------------------------------------------------------ layered ----
+
+@pytest.mark.parametrize("gamma", [(0, 1), (1, 4), (1, 1)])
+def test_layered_matches_oracle(rmat14, gamma):
+    g, (off, nbr, _) = rmat14
+    _, dump, _ = G.run(g, 6, dump=True, method=G.LAYERED, gamma=gamma)
+    want, vols = O.layered(off, nbr, 6, gamma[0], gamma[1])
+    assert_steps_equal(dump, want, "layered%s" % (gamma,))
+    with G.Session(g, method=G.LAYERED, gamma=gamma) as s:
+        s.step(6)
+        assert np.array_equal(s.volume(), vols[-1])
+
+
+def test_layered_weighted(rmat13w):
+    g, (off, nbr, w) = rmat13w
+    _, dump, _ = G.run(g, 5, dump=True, method=G.LAYERED, gamma=(1, 4), weighted=True)
+    want, _ = O.layered(off, nbr, 5, 1, 4, w=w)
+    assert_steps_equal(dump, want, "layered weighted")
+
+
+def test_layered_hub_path():
+    with G.tuning(pack_max_deg=8, wave_max_deg=16, block_max_deg=32, hub_chunk=256, part_keys=256):
+        g = G.Graph.rmat(13, 16, seed=33)
+        off, nbr, _ = g.csr()
+        _, dump, _ = G.run(g, 4, dump=True, method=G.LAYERED, gamma=(1, 1))
+    assert_steps_equal(dump, O.layered(off, nbr, 4, 1, 1)[0], "layered hubs")
+
+
+def test_layered_range_check():
+    g = G.Graph.rmat(10, 8, seed=1)
+    with pytest.raises(G.GLPError):
+        G.run(g, 1, method=G.LAYERED, gamma=(1, 3))   # denominator not a power of two
+
+
+# --------------------------------------------------------------------- SLPA ----
+
+@pytest.mark.parametrize("T", [1, 7])
+def test_slpa_memory_and_post(rmat14, T):
+    g, (off, nbr, _) = rmat14
+    mem_want = O.slpa(off, nbr, seed=11, T=T)
+    with G.Session(g, method=G.SLPA, seed=11, slpa_T=T, slpa_r=(1, 10)) as s:
+        s.step(T)
+        mem = s.slpa_memory(T + 1)
+        dom, ns, sets = s.slpa_post(kmax=8)
+    assert_steps_equal(mem, mem_want, "slpa memory")
+    dw, nw, sw = O.slpa_post(mem_want, 1, 10, 8)
+    assert np.array_equal(dom, dw) and np.array_equal(ns, nw) and np.array_equal(sets, sw)
+    labels, _ = G.run(g, T, method=G.SLPA, seed=11)
+    assert np.array_equal(labels, dw)
+
+
+def test_slpa_lfr_graph():
+    g = G.Graph.lfr(1 << 13, seed=5)
+    off, nbr, _ = g.csr()
+    with G.Session(g, method=G.SLPA, seed=3, slpa_T=10) as s:
+        s.step(10)
+        mem = s.slpa_memory(11)
+    assert_steps_equal(mem, O.slpa(off, nbr, seed=3, T=10), "slpa lfr")
+
+
+# ------------------------------------------------------- approximate counting ----
+
+@pytest.mark.parametrize("rows,width,aliased", [(1, 64, 0), (3, 200, 1), (3, 200, 0), (4, 50, 1),
+                                                (5, 500, 0), (2, 7, 0)])
+def test_countmin_matches_oracle(rmat14, rows, width, aliased):
+    g, (off, nbr, _) = rmat14
+    _, dump, _ = G.run(g, 3, dump=True, method=G.COUNTMIN, cm_rows=rows, cm_width=width,
+                       cm_aliased=aliased)
+    prev = np.arange(len(off) - 1, dtype=np.int32)
+    for t in range(3):
+        prev = O.countmin_step(off, nbr, prev, rows, width, aliased)
+        assert np.array_equal(dump[t], prev), "step %d" % (t + 1)
+
+
+def test_countmin_weighted(rmat13w):
+    g, (off, nbr, w) = rmat13w
+    labels, _ = G.run(g, 2, method=G.COUNTMIN, cm_rows=3, cm_width=100, cm_aliased=1, weighted=True)
+    prev = np.arange(len(off) - 1, dtype=np.int32)
+    for _ in range(2):
+        prev = O.countmin_step(off, nbr, prev, 3, 100, 1, w=w)
+    assert np.array_equal(labels, prev)
+
+
+@pytest.mark.parametrize("method,buffer", [(G.FREQUENT, 0), (G.BUFFER, 1)])
+@pytest.mark.parametrize("cap", [1, 4, 16])
+def test_tables_match_oracle(rmat14, method, buffer, cap):
+    g, (off, nbr, _) = rmat14
+    _, dump, _ = G.run(g, 3, dump=True, method=method, table_cap=cap)
+    prev = np.arange(len(off) - 1, dtype=np.int32)
+    for t in range(3):
+        prev = O.table_step(off, nbr, prev, cap, buffer)
+        assert np.array_equal(dump[t], prev), "step %d" % (t + 1)
+
+
+def test_tables_weighted(rmat13w):
+    g, (off, nbr, w) = rmat13w
+    for method, buf in ((G.FREQUENT, 0), (G.BUFFER, 1)):
+        labels, _ = G.run(g, 2, method=method, table_cap=3, weighted=True)
+        prev = np.arange(len(off) - 1, dtype=np.int32)
+        for _ in range(2):
+            prev = O.table_step(off, nbr, prev, 3, buf, w=w)
+        assert np.array_equal(labels, prev)
+
+
+# --------------------------------------------------------------- user hooks ----
+
+def test_user_hooks_are_classic(rmat14):
+    g, (off, nbr, _) = rmat14
+    _, dump, _ = G.run(g, 5, dump=True, method=G.USER)
+    assert_steps_equal(dump, O.classic(off, nbr, 5), "user hooks")
+    _, dump2, _ = G.run(g, 5, dump=True, method=G.USER, gather=True)
+    assert np.array_equal(dump, dump2)
+
+
+# ------------------------------------------------------------------ batched ----
+
+def mix_graphs(count, seed, nmax=6000):
+    gs = []
+    for i in range(count):
+        p = G.mix_params(seed, i, nmin=200, nmax=nmax)
+        gs.append(G.Graph.planted(p["n"], p["k"], p["deg_in"], p["deg_out"], seed=p["seed"]))
+    return gs
+
+
+@pytest.mark.parametrize("method,kw", [(G.CLASSIC, {}), (G.LAYERED, {"gamma": (1, 4)}),
+                                       (G.FREQUENT, {"table_cap": 8})])
+def test_batch_equals_alone(method, kw):
+    gs = mix_graphs(24, 5)
+    gs.append(G.Graph.rmat(12, 16, seed=2))     # one graph with hubs
+    b = G.Batch(gs, method=method, **kw)
+    for mode in (0, 1, 2):
+        b.reset()
+        b.run(6, mode=mode)
+        for i, g in enumerate(gs):
+            alone, _ = G.run(g, 6, method=method, **kw)
+            assert np.array_equal(b.labels(i), alone), "mode %d graph %d" % (mode, i)
+    b.free()
+
+
+def test_batch_rejects_stream_methods():
+    gs = mix_graphs(2, 1)
+    with pytest.raises(G.GLPError):
+        G.Batch(gs, method=G.SLPA)
+
+
+# ------------------------------------------------------------------- served ----
+
+def test_server_results_equal_alone_in_the_mix():
+    rng = np.random.default_rng(3)
+    graphs, params, steps = [], [], []
+    for i in range(12):
+        scale = int(rng.integers(8, 13))
+        g = G.Graph.rmat(scale, int(rng.integers(8, 17)), seed=100 + i, wmax=8)
+        layered = i % 2 == 1
+        p = G.make_params(method=G.LAYERED if layered else G.CLASSIC, gamma=(1, 4) if layered else (0, 1),
+                          weighted=True)
+        graphs.append(g)
+        params.append(p)
+        steps.append(int(rng.integers(3, 9)))
+    sv = G.Server(budget_bytes=64 << 20, max_slots=4)   # small budget: jobs queue for memory
+    ids = [sv.submit(g, s, params=p) for g, s, p in zip(graphs, steps, params)]
+    for jid, g, s, p in zip(ids, graphs, steps, params):
+        got, st = sv.wait(jid, g.n)
+        alone, _ = G.run(g, s, params=p)
+        assert st.state == 2 and st.steps == s
+        assert np.array_equal(got, alone)
+    sv.close()
+
+
+def test_server_budget_and_cancel():
+    sv = G.Server(budget_bytes=1 << 20, max_slots=2)
+    big = G.Graph.rmat(14, 16, seed=1)
+    with pytest.raises(G.GLPError) as e:
+        sv.submit(big, 3)
+    assert e.value.code == -8
+    small = G.Graph.rmat(8, 8, seed=2)
+    a = sv.submit(small, 200)
+    b = sv.submit(small, 2)
+    sv.cancel(a)
+    got, st = sv.wait(b, small.n)
+    alone, _ = G.run(small, 2)
+    assert np.array_equal(got, alone)
+    assert sv.poll(a).state == 3
+    sv.close()

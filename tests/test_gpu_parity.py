@@ -84,6 +84,27 @@ def test_classic_tier_boundaries():
     assert_steps_equal(gpu_steps(g, 5), O.classic(off, nbr, 5))
 
 
+@pytest.mark.parametrize("limit", [0, 8])
+def test_hub_partition_and_halving(limit):
+    """Tiny thresholds push most vertices through the hub passes; part_limit=8
+    forces nearly every hash range to overflow and be halved."""
+    with G.tuning(pack_max_deg=8, wave_max_deg=16, block_max_deg=32, hub_chunk=256,
+                  part_keys=256, part_limit=limit):
+        g = G.Graph.rmat(14, 16, seed=21)
+        off, nbr, _ = g.csr()
+        labels, dump, timing = G.run(g, 5, dump=True)
+    assert timing["tier_vertices"]["hub"] > 1000
+    assert_steps_equal(dump, O.classic(off, nbr, 5))
+
+
+def test_hub_weighted_small_tiers():
+    with G.tuning(pack_max_deg=4, wave_max_deg=8, block_max_deg=16, hub_chunk=256, part_keys=256):
+        g = G.Graph.rmat(13, 16, seed=22, wmax=8)
+        off, nbr, w = g.csr()
+        got = gpu_steps(g, 4, weighted=True)
+    assert_steps_equal(got, O.classic(off, nbr, 4, w=w))
+
+
 def test_classic_multigraph_and_isolated():
     rng = np.random.default_rng(1)
     n = 3000
