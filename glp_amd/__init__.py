@@ -148,6 +148,20 @@ def lib():
     return _lib
 
 
+_sig("glp_autotune", ctypes.c_int, _vp, ctypes.POINTER(Params), ctypes.c_int32, ctypes.c_char_p,
+     ctypes.POINTER(ctypes.c_double))
+_sig("glp_session_set_gamma", ctypes.c_int, _vp, ctypes.c_int32, ctypes.c_int32)
+
+
+def autotune(graph, params, steps=5, out_path=None):
+    """Times candidate tunings on `graph`, makes the fastest the process
+    default (and writes it to out_path); returns its time for `steps` steps."""
+    best = ctypes.c_double(0)
+    _check(_lib.glp_autotune(graph.handle, ctypes.byref(params), steps,
+                             os.fsencode(out_path) if out_path else None, ctypes.byref(best)))
+    return best.value
+
+
 def tuning_get(key):
     v = ctypes.c_int32()
     _check(_lib.glp_tuning_get(key.encode(), ctypes.byref(v)))
@@ -389,6 +403,9 @@ class Session:
 
     def sync(self):
         _check(_lib.glp_session_sync(self._h))
+
+    def set_gamma(self, num, den):
+        _check(_lib.glp_session_set_gamma(self._h, num, den))
 
     def labels(self):
         out = np.zeros(max(self.n, 1), dtype=np.int32)

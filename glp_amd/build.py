@@ -16,7 +16,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("GLP_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip", "counting.hip", "session.hip", "batch.hip", "server.hip", "plan.hip"]
+HIP_SOURCES = ["kernels.hip", "counting.hip", "session.hip", "batch.hip", "server.hip", "plan.hip",
+               "autotune.hip"]
 CPP_SOURCES = ["graph_io.cpp", "gen.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]

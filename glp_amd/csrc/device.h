@@ -72,11 +72,14 @@ struct DeviceGraph {
     int2* d_hub_parts = nullptr;
     double upload_ms = 0, plan_ms = 0;
     size_t bytes = 0;
+    bool owns_csr = true;  // false: off/nbr/w borrowed from another DeviceGraph
     ~DeviceGraph();
 };
 int device_graph_create(const HostCSR& g, const Tuning& t, hipStream_t s, DeviceGraph** out);
 // Uploads a graph with an already built plan (synchronises stream s).
 int device_graph_upload(const HostCSR& g, Plan&& plan, hipStream_t s, DeviceGraph** out);
+// Uploads dg->plan's lists into dg's device arrays (synchronises stream s).
+int device_plan_upload(DeviceGraph* dg, hipStream_t s);
 // HBM bytes a device graph of this shape needs (planning estimate).
 size_t device_graph_bytes(const HostCSR& g, const Plan& p);
 

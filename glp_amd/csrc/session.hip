@@ -22,9 +22,11 @@ static double now_ms() {
 }
 
 DeviceGraph::~DeviceGraph() {
-    (void)hipFree(off);
-    (void)hipFree(nbr);
-    (void)hipFree(w);
+    if (owns_csr) {
+        (void)hipFree(off);
+        (void)hipFree(nbr);
+        (void)hipFree(w);
+    }
     (void)hipFree(d_groups);
     (void)hipFree(d_wave);
     for (int c = 0; c < kMaxBlockClasses; ++c) (void)hipFree(d_block[c]);
@@ -75,10 +77,21 @@ int device_graph_upload(const HostCSR& g, Plan&& plan, hipStream_t s, DeviceGrap
     int rc;
     double t1 = now_ms();
     size_t bytes = 0;
-    const Plan& p = dg->plan;
     if ((rc = upload(&dg->off, g.offsets.data(), (size_t)g.n + 1, s, bytes))) return rc;
     if ((rc = upload(&dg->nbr, g.neighbors.data(), (size_t)g.m, s, bytes))) return rc;
     if (g.weighted() && (rc = upload(&dg->w, g.weights.data(), (size_t)g.m, s, bytes))) return rc;
+    if ((rc = device_plan_upload(dg.get(), s))) return rc;
+    double t2 = now_ms();
+    dg->upload_ms = t2 - t1;
+    dg->bytes += bytes;
+    *out = dg.release();
+    return GLP_OK;
+}
+
+int device_plan_upload(DeviceGraph* dg, hipStream_t s) {
+    int rc;
+    size_t bytes = 0;
+    const Plan& p = dg->plan;
     if ((rc = upload(&dg->d_groups, p.groups.data(), p.groups.size(), s, bytes))) return rc;
     if ((rc = upload(&dg->d_wave, p.wave.data(), p.wave.size(), s, bytes))) return rc;
     for (int c = 0; c < p.nclasses; ++c)
@@ -89,10 +102,7 @@ int device_graph_upload(const HostCSR& g, Plan&& plan, hipStream_t s, DeviceGrap
     if ((rc = upload(&dg->d_hub_base, p.hub_base.data(), p.hub_base.size(), s, bytes))) return rc;
     if ((rc = upload(&dg->d_hub_parts, p.hub_parts.data(), p.hub_parts.size(), s, bytes))) return rc;
     GLP_HIP(hipStreamSynchronize(s));
-    double t2 = now_ms();
-    dg->upload_ms = t2 - t1;
-    dg->bytes = bytes;
-    *out = dg.release();
+    dg->bytes += bytes;
     return GLP_OK;
 }
 
@@ -513,6 +523,24 @@ extern "C" int glp_session_step_graph(glp_session* s, int32_t steps) {
     GLP_GUARD(s);
     return session_step_graph(s, steps);
 }
+extern "C" int glp_session_set_gamma(glp_session* s, int32_t gamma_num, int32_t gamma_den) {
+    GLP_GUARD(s);
+    if (s->method != Method::Layered) return set_error(GLP_E_STATE, "gamma exists for layered sessions only");
+    glp_params q = s->p;
+    q.gamma_num = gamma_num;
+    q.gamma_den = gamma_den;
+    int rc = check_params(s->g->csr, &q);
+    if (rc != GLP_OK) return rc;
+    s->p = q;
+    // captured step graphs baked the old scores in
+    for (auto& ge : s->graph_exec)
+        if (ge) {
+            (void)hipGraphExecDestroy(ge);
+            ge = nullptr;
+        }
+    return GLP_OK;
+}
+
 extern "C" int glp_session_sync(glp_session* s) {
     GLP_GUARD(s);
     return session_sync(s);

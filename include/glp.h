@@ -101,6 +101,13 @@ int glp_device_count(int32_t* count);
  * part_limit, block_threads.                                                 */
 int glp_tuning_set(const char* key, int32_t value);
 int glp_tuning_get(const char* key, int32_t* value);
+/* Autotuner (no reference counterpart; replaces hand-picked constants):
+ * times `steps` steps from L_0 on g under candidate tunings (coordinate
+ * descent over the keys above), makes the fastest the process default and,
+ * when out_path is not NULL, writes it as key=value lines.  best_ms receives
+ * the winner's time for the timed steps.                                    */
+int glp_autotune(glp_graph* g, const glp_params* p, int32_t steps, const char* out_path,
+                 double* best_ms);
 
 /* ---- graphs ----------------------------------------------------------- */
 /* Replaces header/graph.h:69-147 (CSRGraph ctor) with :227-258
@@ -149,6 +156,10 @@ int glp_session_reset(glp_session* s);
 int glp_session_step(glp_session* s, int32_t steps);
 /* Captures one step in a HIP graph and replays it `steps` times.            */
 int glp_session_step_graph(glp_session* s, int32_t steps);
+/* Layered: changes gamma for the following steps; labels and volumes carry
+ * over (the reference's per-round delta schedule, prop/MultiLprop.cuh:308-314,
+ * with gamma = 1/delta).                                                     */
+int glp_session_set_gamma(glp_session* s, int32_t gamma_num, int32_t gamma_den);
 /* Waits for the session's stream.                                           */
 int glp_session_sync(glp_session* s);
 /* Copies the current labels (n int32) to host; blocks.                      */
