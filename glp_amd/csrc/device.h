@@ -39,7 +39,8 @@ constexpr int kMaxBlockClasses = 5;
 // Host-built tier plan.  Vertex lists hold ids; groups hold (first id, count).
 struct Plan {
     Tuning tune;
-    std::vector<int2> groups;                 // packed tier
+    std::vector<int2> groups;                 // packed tier (edge-less groups last)
+    int64_t groups_with_edges = 0;            // groups before the edge-less ones
     std::vector<int32_t> wave;                // wave tier
     int32_t nclasses = 0;
     int32_t class_cap[kMaxBlockClasses] = {};  // LDS slots of each block class

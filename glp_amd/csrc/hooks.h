@@ -55,6 +55,9 @@ __host__ __device__ __forceinline__ int32_t packed_label(uint64_t p) {
 // header/kernel.cuh:1724-1729 packed (count<<32)+key) --------------------
 struct ClassicHooks {
     static constexpr bool kPick = false;
+    // an isolated vertex keeps its label and has no side effects, so groups of
+    // isolated vertices need no work once both label buffers hold the label
+    static constexpr bool kSkipIsolated = true;
     __device__ static __forceinline__ int32_t pick_label(const StepArgs& a, int32_t vid) {
         return a.keep[vid];
     }
@@ -83,6 +86,7 @@ struct ClassicHooks {
 // (prop/MultiLprop.cuh:313-314, header/kernel.cuh:2579 delta*count - gcounter)
 struct LayeredHooks {
     static constexpr bool kPick = false;
+    static constexpr bool kSkipIsolated = false;  // isolated vertices still add to V
     __device__ static __forceinline__ int32_t pick_label(const StepArgs& a, int32_t vid) {
         return a.keep[vid];
     }
@@ -112,6 +116,7 @@ struct LayeredHooks {
 // speaker choice is the pick kernel (slpa_speak).  prop/SLPAprop.cuh:337-398.
 struct SlpaHooks {
     static constexpr bool kPick = true;
+    static constexpr bool kSkipIsolated = false;  // every memory row is written
     // Speaker rule: u shows M_u[R(seed,t,u) mod t] (memory rows 0..t-1).
     __device__ static __forceinline__ int32_t pick_label(const StepArgs& a, int32_t vid) {
         uint64_t x = (a.seed << 40) ^ ((uint64_t)(uint32_t)a.t << 32) ^ (uint64_t)(uint32_t)vid;

@@ -621,8 +621,8 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
         if (ev) (void)hipEventRecord(ev[2 * tier + edge], s);
     };
     mark(GLP_TIER_PACKED, 0);
-    if (!p.groups.empty()) {
-        const int32_t ng = (int32_t)p.groups.size();
+    const int32_t ng = (int32_t)(H::kSkipIsolated ? p.groups_with_edges : (int64_t)p.groups.size());
+    if (ng > 0) {
         hipLaunchKernelGGL((k_packed<H, W, G>), dim3(grid_for(ng, 4)), dim3(256), 0, s, a,
                            g.d_groups, ng);
     }

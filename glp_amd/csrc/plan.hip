@@ -160,6 +160,18 @@ int build_plan(const HostCSR& g, const Tuning& t, Plan& p) {
         }
     }
     close_group();
+    // groups holding only isolated vertices go last: classic LP skips them
+    // (such a vertex keeps its label and both label buffers already hold it)
+    {
+        std::vector<int2> with_edges, isolated;
+        for (const int2& gr : p.groups) {
+            const bool any = g.offsets[gr.x + gr.y] > g.offsets[gr.x];
+            (any ? with_edges : isolated).push_back(gr);
+        }
+        p.groups_with_edges = (int64_t)with_edges.size();
+        p.groups = std::move(with_edges);
+        p.groups.insert(p.groups.end(), isolated.begin(), isolated.end());
+    }
     if (hub_chunks > INT32_MAX) return set_error(GLP_E_RANGE, "too many hub chunks");
     return GLP_OK;
 }
