@@ -58,6 +58,134 @@ def cpu_baseline(off, nbr, target_s, threads, max_steps):
                       % (steps, len(off) - 1, m, secs, secs1)}
 
 
+def bench_batched(args, G):
+    """1,024 planted-partition graphs (n log-uniform in [1000, 262144], one
+    community per ~1000 vertices, average degree U[4,64], inter-community
+    fraction U[.1,.4]), classic LP, 20 steps: the packed batch replayed from
+    a HIP graph against one plain launch sequence per graph."""
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    log("generating %d mix graphs" % args.graphs)
+    graphs = []
+    for i in range(args.graphs):
+        p = G.mix_params(args.seed, i)
+        graphs.append(G.Graph.planted(p["n"], p["k"], p["deg_in"], p["deg_out"], seed=p["seed"],
+                                      threads=threads))
+    n_tot = sum(g.n for g in graphs)
+    m_tot = sum(g.m for g in graphs)
+    log("mix: %d graphs, %d vertices, %d edges" % (len(graphs), n_tot, m_tot))
+    b = G.Batch(graphs, method=G.CLASSIC)
+    tuned_ms = b.autotune(3) if not args.no_autotune else None
+    log("autotuned" if tuned_ms else "not autotuned")
+    steps = args.steps
+    # plain: one session per graph, stepped one graph after another
+    b.run(1, mode=2)
+    b.reset()
+    plain_ms = b.run(steps, mode=2)
+    plain_labels = [b.labels(i) for i in range(len(graphs))]
+    log("plain %.1f ms" % plain_ms)
+    # batched: the packed batch, steps replayed from a captured HIP graph
+    b.run(1, mode=1)
+    b.reset()
+    batch_ms = b.run(steps, mode=1)
+    same = all(np.array_equal(b.labels(i), plain_labels[i]) for i in range(len(graphs)))
+    log("batched %.1f ms, labels identical: %s" % (batch_ms, same))
+    b.reset()
+    eager_ms = b.run(steps, mode=0)
+    gps = len(graphs) / (batch_ms / 1e3)
+    plain_gps = len(graphs) / (plain_ms / 1e3)
+    return {
+        "metric": "batched graphs/s vs plain (1,024-graph planted-partition mix, classic LP, %d steps)" % steps,
+        "value": round(gps, 2), "unit": "graphs/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": round(batch_ms / steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "batched classic LP over %d planted-partition graphs" % len(graphs),
+                   "graphs": len(graphs), "n_total": n_tot, "m_total": m_tot, "seed": args.seed,
+                   "autotuned": tuned_ms is not None},
+        "plain_graphs_per_s": round(plain_gps, 2), "speedup_vs_plain": round(gps / plain_gps, 3),
+        "batched_eager_ms": round(eager_ms, 2), "plain_ms": round(plain_ms, 2),
+        "batched_ms": round(batch_ms, 2), "labels_identical": bool(same),
+        "gteps_batched": round(m_tot * steps / (batch_ms / 1e3) / 1e9, 3),
+    }
+
+
+def bench_served(args, G):
+    """Poisson arrivals of weighted classic / layered (gamma 1/4) LP jobs on
+    seeded RMAT graphs (scale U{lo..hi}, edge factor U{8..32}, weights
+    U[1,8]), 10 steps each, under a 200 GB HBM budget; p50/p99 latency."""
+    rng = np.random.default_rng(args.seed)
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    lo, hi = args.serve_scales
+    specs = []
+    for i in range(args.jobs):
+        scale = int(rng.integers(lo, hi + 1))
+        ef = int(rng.integers(8, 33))
+        layered = bool(rng.integers(0, 2))
+        specs.append((scale, ef, layered))
+    log("generating %d job graphs (scales %d..%d)" % (args.jobs, lo, hi))
+    graphs = [G.Graph.rmat(s, ef, seed=args.seed * 100003 + i, wmax=8, threads=threads)
+              for i, (s, ef, _) in enumerate(specs)]
+    params = [G.make_params(method=G.LAYERED if lay else G.CLASSIC, gamma=(1, 4) if lay else (0, 1),
+                            weighted=True) for (_, _, lay) in specs]
+    m_tot = sum(g.m for g in graphs)
+    sv = G.Server(budget_bytes=200 << 30, max_slots=args.slots)
+    gaps = rng.exponential(1.0 / args.rate, size=args.jobs)
+    t_start = time.perf_counter()
+    due = t_start
+    ids = []
+    for i, g in enumerate(graphs):
+        due += gaps[i]
+        delay = due - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(sv.submit(g, 10, params=params[i]))
+    results = [sv.wait(j, g.n) for j, g in zip(ids, graphs)]
+    lat = np.array([st.done_ms - st.submit_ms for _, st in results])
+    first = min(st.submit_ms for _, st in results)
+    last = max(st.done_ms for _, st in results)
+    # each job alone must give the same labels as in the mix
+    check = list(range(0, args.jobs, max(1, args.jobs // 16)))
+    same = all(np.array_equal(results[i][0], G.run(graphs[i], 10, params=params[i])[0]) for i in check)
+    sv.close()
+    jobs_s = args.jobs / ((last - first) / 1e3)
+    return {
+        "metric": "served LP jobs/s with p50/p99 latency (Poisson arrivals, 1 GPU)",
+        "value": round(jobs_s, 2), "unit": "jobs/s", "n_gpus": 1, "steps": 10, "warmup": 0,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "%d weighted classic/layered(gamma=1/4) RMAT jobs, scale U{%d..%d}, "
+                               "edge factor U{8..32}, weights U[1,8], 10 steps, Poisson rate %.0f/s, "
+                               "200 GB budget, %d slots" % (args.jobs, lo, hi, args.rate, args.slots),
+                   "jobs": args.jobs, "m_total": int(m_tot)},
+        "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "mean_ms": round(float(lat.mean()), 3), "edges_steps_per_s": round(m_tot * 10 / ((last - first) / 1e3), 1),
+        "results_equal_alone": bool(same), "checked_jobs": len(check),
+    }
+
+
+def bench_copy(args):
+    """Calibration for rocprof byte counters: device-to-device copies of a
+    known size (the runtime's copy kernel), so FETCH_SIZE / WRITE_SIZE per
+    dispatch can be compared with the bytes actually moved."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    size = args.copy_mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(size)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(size)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(size)) == 0
+    reps = 5
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(size), 3) == 0   # device to device
+    hip.hipDeviceSynchronize()
+    dt = (time.perf_counter() - t0) / reps
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return {"metric": "copy calibration", "value": round(2 * size / dt / 1e9, 1), "unit": "GB/s",
+            "bytes_read_per_copy": size, "bytes_written_per_copy": size, "copies": reps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +197,16 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph-replay", action="store_true", help="replay steps from a HIP graph")
+    ap.add_argument("--gather-steps", type=int, default=3,
+                    help="steps of the separate-gather mode timed after the main run (0: skip)")
+    ap.add_argument("--workload", choices=["rmat", "batched", "served", "copy"], default="rmat")
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--graphs", type=int, default=1024)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--jobs", type=int, default=200)
+    ap.add_argument("--rate", type=float, default=100.0)
+    ap.add_argument("--slots", type=int, default=64)
+    ap.add_argument("--serve-scales", type=int, nargs=2, default=[12, 17])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -79,6 +217,15 @@ def main():
     if G.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "batched":
+        print(json.dumps(bench_batched(args, G)), flush=True)
+        return
+    if args.workload == "served":
+        print(json.dumps(bench_served(args, G)), flush=True)
+        return
+    if args.workload == "copy":
+        print(json.dumps(bench_copy(args)), flush=True)
+        return
     threads = min(len(os.sched_getaffinity(0)), 16)
     log("generating RMAT scale %d ef %d seed %d (%d threads)" % (args.scale, args.edge_factor,
                                                                args.seed, threads))
@@ -133,6 +280,25 @@ def main():
                      "frac": round(whole / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
     sess.free()
 
+    # the separate gather pass (LoadNeighbor as its own kernel): 12 B per edge
+    gather = None
+    if args.gather_steps > 0:
+        gs = G.Session(g, G.make_params(method=G.CLASSIC, gather=True, profile=True))
+        gs.step(1)
+        gs.sync()
+        g0 = gs.timing()
+        gs.step(args.gather_steps)
+        gs.sync()
+        g1 = gs.timing()
+        n_l = g1["tier_launches"]["gather"] - g0["tier_launches"]["gather"]
+        gms = (g1["tier_ms"]["gather"] - g0["tier_ms"]["gather"]) / max(n_l, 1)
+        gather = {"kernel": "k_gather", "kernel_ms": round(gms, 4), "algorithmic_bytes": 12 * m,
+                  "achieved_GBs": round(12 * m / (gms / 1e3) / 1e9, 1),
+                  "frac": round(12 * m / (gms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                  "step_ms": round((g1["steps_ms"] - g0["steps_ms"]) / args.gather_steps, 4)}
+        log("gather kernel %.3f ms -> %.1f GB/s" % (gms, gather["achieved_GBs"]))
+        gs.free()
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (%d threads, ~%.0f s)" % (threads, args.cpu_seconds))
@@ -153,6 +319,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "step_roofline": step_roofline,
+        "gather": gather,
         "tiers_ms": {k: round(v["ms"], 4) for k, v in tiers.items()},
         "changes_per_step": changes[-args.steps:],
     }

@@ -47,7 +47,7 @@ class Params(ctypes.Structure):
                 ("cm_rows", ctypes.c_int32), ("cm_width", ctypes.c_int32),
                 ("cm_aliased", ctypes.c_int32), ("table_cap", ctypes.c_int32),
                 ("gather", ctypes.c_int32), ("profile", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 6)]
+                ("frontier", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
 
 
 class Timing(ctypes.Structure):
@@ -128,6 +128,7 @@ _sig("glp_batch_run", ctypes.c_int, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.
 _sig("glp_batch_labels", ctypes.c_int, _vp, ctypes.c_int32, _c_i32p)
 _sig("glp_batch_reset", ctypes.c_int, _vp)
 _sig("glp_batch_free", None, _vp)
+_sig("glp_batch_autotune", ctypes.c_int, _vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double))
 _sig("glp_server_create", ctypes.c_int, ctypes.c_uint64, ctypes.c_int32, _pp)
 _sig("glp_server_submit", ctypes.c_int, _vp, _vp, ctypes.POINTER(Params), ctypes.c_int32, _c_i64p)
 _sig("glp_server_poll", ctypes.c_int, _vp, ctypes.c_int64, ctypes.POINTER(JobStats))
@@ -214,7 +215,8 @@ def device_count():
 
 
 def make_params(method=CLASSIC, weighted=False, gamma=(0, 1), seed=1, slpa_T=20, slpa_r=(1, 10),
-                cm_rows=3, cm_width=200, cm_aliased=False, table_cap=16, gather=False, profile=False):
+                cm_rows=3, cm_width=200, cm_aliased=False, table_cap=16, gather=False, profile=False,
+                frontier=False):
     p = Params()
     _lib.glp_params_default(ctypes.byref(p))
     p.method = int(method)
@@ -227,6 +229,7 @@ def make_params(method=CLASSIC, weighted=False, gamma=(0, 1), seed=1, slpa_T=20,
     p.table_cap = int(table_cap)
     p.gather = int(bool(gather))
     p.profile = int(bool(profile))
+    p.frontier = int(bool(frontier))
     return p
 
 
@@ -495,6 +498,11 @@ class Batch:
 
     def reset(self):
         _check(_lib.glp_batch_reset(self._h))
+
+    def autotune(self, steps=3):
+        best = ctypes.c_double(0)
+        _check(_lib.glp_batch_autotune(self._h, steps, ctypes.byref(best)))
+        return best.value
 
     def labels(self, i):
         n = self.graphs[i].n

@@ -144,4 +144,15 @@ extern "C" int glp_batch_reset(glp_batch* b) {
     return GLP_OK;
 }
 
+extern "C" int glp_batch_autotune(glp_batch* b, int32_t steps, double* best_ms) {
+    clear_error();
+    if (!b || steps < 1) return set_error(GLP_E_ARG, "bad batch autotune arguments");
+    // the union session's scratch was sized by the old plan: rebuild it
+    delete b->sess;
+    b->sess = nullptr;
+    int rc = glp_autotune(b->uni.get(), &b->p, steps, nullptr, best_ms);
+    if (rc != GLP_OK) return rc;
+    return session_create(b->uni.get(), &b->p, &b->sess);
+}
+
 extern "C" void glp_batch_free(glp_batch* b) { delete b; }

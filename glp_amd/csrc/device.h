@@ -91,6 +91,7 @@ struct HubScratch {
     unsigned long long* pairs = nullptr;
     uint32_t* chunk_cnt = nullptr;
     unsigned long long* best = nullptr;
+    uint32_t* changed = nullptr;  // frontier: hub's label changed this step
 };
 
 // Hub-tier launch arguments (plan lists plus session scratch).
@@ -136,7 +137,11 @@ int launch_gather(const StepArgs& a, int64_t m, int32_t* adj, hipStream_t s);
 int launch_fill_i32(int32_t* p, int32_t value, int64_t count, hipStream_t s);
 int launch_iota(int32_t* p, int64_t count, hipStream_t s);
 int launch_end_step(unsigned long long* acc, unsigned long long* ring, int32_t* step_ctr,
-                    int32_t ring_size, hipStream_t s);
+                    int32_t ring_size, hipStream_t s, int32_t* fctl = nullptr);
+// Fills count bytes (count a multiple of 4, p 4-byte aligned) with zero.
+inline int launch_zero_bytes(void* p, int64_t count, hipStream_t s) {
+    return launch_fill_i32(static_cast<int32_t*>(p), 0, count / 4, s);
+}
 
 // Folded step for nj jobs with layered/classic hooks (gamma 0 = classic).
 // jobs and cum are device arrays; grid (host) holds each launch's size.

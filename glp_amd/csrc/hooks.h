@@ -39,6 +39,13 @@ struct StepArgs {
     int32_t t;               // step number (1-based)
     const int32_t* mem;      // SLPA memory, (T+1) x n
     uint64_t seed;           // SLPA stream seed
+    // frontier (classic only; null when off): act_r[v] = some neighbour of v
+    // changed in the previous step; act_w collects that for the next step;
+    // fctl[0] = 1: this step recomputes every vertex, fctl[1] = 1: this step
+    // marks act_w (both set on the device by k_end_step)
+    const uint8_t* act_r;
+    uint8_t* act_w;
+    const int32_t* fctl;
 };
 
 __host__ __device__ __forceinline__ uint32_t order_i32(int32_t s) {
@@ -58,6 +65,8 @@ struct ClassicHooks {
     // an isolated vertex keeps its label and has no side effects, so groups of
     // isolated vertices need no work once both label buffers hold the label
     static constexpr bool kSkipIsolated = true;
+    // a vertex whose neighbours all kept their labels keeps its own: exact
+    static constexpr bool kFrontier = true;
     __device__ static __forceinline__ int32_t pick_label(const StepArgs& a, int32_t vid) {
         return a.keep[vid];
     }
@@ -87,6 +96,7 @@ struct ClassicHooks {
 struct LayeredHooks {
     static constexpr bool kPick = false;
     static constexpr bool kSkipIsolated = false;  // isolated vertices still add to V
+    static constexpr bool kFrontier = false;      // V changes every label's score
     __device__ static __forceinline__ int32_t pick_label(const StepArgs& a, int32_t vid) {
         return a.keep[vid];
     }
@@ -117,6 +127,7 @@ struct LayeredHooks {
 struct SlpaHooks {
     static constexpr bool kPick = true;
     static constexpr bool kSkipIsolated = false;  // every memory row is written
+    static constexpr bool kFrontier = false;      // speakers change every step
     // Speaker rule: u shows M_u[R(seed,t,u) mod t] (memory rows 0..t-1).
     __device__ static __forceinline__ int32_t pick_label(const StepArgs& a, int32_t vid) {
         uint64_t x = (a.seed << 40) ^ ((uint64_t)(uint32_t)a.t << 32) ^ (uint64_t)(uint32_t)vid;

@@ -87,6 +87,30 @@ __device__ __forceinline__ void count_changes(const StepArgs& a, bool changed) {
         atomicAdd(&a.changes[blockIdx.x & (kChangeSlots - 1)], (unsigned long long)__popcll(bal));
 }
 
+// Frontier helpers (classic LP; header/kernel.cuh:21-35's if_update idea):
+// with the frontier on, a vertex whose neighbours all kept their labels in
+// the previous step keeps its own and is not recomputed; its buffers already
+// agree (a vertex that changed marks itself).  A changed vertex marks itself
+// and its neighbours for the next step.
+template <class H>
+__device__ __forceinline__ bool frontier_on(const StepArgs& a) {
+    return H::kFrontier && a.act_r != nullptr && a.fctl[0] == 0;
+}
+template <class H>
+__device__ __forceinline__ bool is_active(const StepArgs& a, int32_t v) {
+    return !frontier_on<H>(a) || a.act_r[v] != 0;
+}
+template <class H>
+__device__ __forceinline__ bool marking(const StepArgs& a) {
+    return H::kFrontier && a.act_w != nullptr && a.fctl[1] != 0;
+}
+// Marks v and its neighbours [b, e), `stride` threads cooperating.
+__device__ __forceinline__ void mark_vertex(const StepArgs& a, int32_t v, int32_t b, int32_t e,
+                                            int tid, int stride) {
+    if (tid == 0) a.act_w[v] = 1;
+    for (int32_t j = b + tid; j < e; j += stride) a.act_w[a.nbr[j]] = 1;
+}
+
 // Each tier is a __device__ body taking its workgroup index `blk`; the
 // __global__ wrappers below run it for one job (blk = blockIdx.x) or for a
 // table of jobs folded into one launch (blk = index inside the owning job).
@@ -96,16 +120,21 @@ template <class H, bool W, bool G>
 __device__ __forceinline__ void packed_body(const StepArgs& a, const int2* __restrict__ groups,
                                             int32_t ngroups, uint32_t blk) {
     const int lane = lane_id();
-    const int32_t gid = (int32_t)((blk * 256u + threadIdx.x) >> 6);
-    if (gid >= ngroups) return;  // wave-uniform
+    // wave-uniform group index (readfirstlane keeps the group loads scalar)
+    const int32_t gid = __builtin_amdgcn_readfirstlane((int32_t)((blk * 256u + threadIdx.x) >> 6));
+    if (gid >= ngroups) return;
     const int2 gr = groups[gid];
     const int32_t v0 = gr.x, vc = gr.y;
-    const int32_t base = a.off[v0];
-    int32_t vb = 0, ve = 0;
-    if (lane < vc) {
-        vb = a.off[v0 + lane] - base;
-        ve = a.off[v0 + lane + 1] - base;
+    if (frontier_on<H>(a)) {
+        const bool act = lane < vc && a.act_r[v0 + lane] != 0;
+        if (__ballot(act) == 0ull) return;  // no vertex of the group needs work
     }
+    const int32_t base = a.off[v0];
+    // vertex lane i: its edges are lanes [vb, ve) of the group
+    int32_t ve = 0;
+    if (lane < vc) ve = a.off[v0 + lane + 1] - base;
+    int32_t vb = shfl_i32(ve, lane > 0 ? lane - 1 : 0);
+    if (lane == 0) vb = 0;
     const int32_t nE = shfl_i32(ve, vc - 1);
     // owner of edge lane `lane`: smallest vertex slot i with ve_i > lane
     int lo = 0, hi = vc - 1;
@@ -148,6 +177,11 @@ __device__ __forceinline__ void packed_body(const StepArgs& a, const int2* __res
         H::update_vertex(a, v, nl, has ? best : 0ull);
         changed = nl != old;
     }
+    if (marking<H>(a)) {
+        if (changed) a.act_w[v0 + lane] = 1;
+        const bool owner_changed = shfl_i32(changed ? 1 : 0, own) != 0;
+        if (live && owner_changed) a.act_w[a.nbr[base + lane]] = 1;
+    }
     count_changes(a, changed);
 }
 
@@ -161,6 +195,7 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int32_t* __re
     const int32_t idx = (int32_t)blk * 4 + wv;
     if (idx >= nv) return;  // wave-uniform
     const int32_t v = verts[idx];
+    if (!is_active<H>(a, v)) return;
     const int32_t b = a.off[v], e = a.off[v + 1];
     const uint32_t cap = min(max(next_pow2(2u * (uint32_t)(e - b)), 64u), (uint32_t)kWaveCap);
     const uint32_t mask = cap - 1u;
@@ -186,6 +221,7 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int32_t* __re
         H::update_vertex(a, v, nl, best);
         changed = nl != old;
     }
+    if (marking<H>(a) && shfl_i32(changed ? 1 : 0, 0) != 0) mark_vertex(a, v, b, e, lane, 64);
     count_changes(a, changed);
 }
 
@@ -208,9 +244,11 @@ __device__ __forceinline__ void block_body(const StepArgs& a, const int32_t* __r
                                            uint32_t cap_max, uint32_t blk) {
     extern __shared__ uint32_t s_dyn[];
     __shared__ unsigned long long s_red[NT / 64];
+    __shared__ uint32_t s_changed;
     uint32_t* K = s_dyn;
     uint32_t* C = s_dyn + cap_max;
     const int32_t v = verts[blk];
+    if (!is_active<H>(a, v)) return;  // block-uniform
     const int32_t b = a.off[v], e = a.off[v + 1];
     const uint32_t cap = min(max(next_pow2(2u * (uint32_t)(e - b)), 64u), cap_max);
     const uint32_t mask = cap - 1u;
@@ -234,8 +272,13 @@ __device__ __forceinline__ void block_body(const StepArgs& a, const int32_t* __r
             const int32_t nl = packed_label(best);
             H::update_vertex(a, v, nl, best);
             changed = nl != old;
+            s_changed = changed ? 1u : 0u;
         }
         count_changes(a, changed);
+    }
+    if (marking<H>(a)) {
+        __syncthreads();
+        if (s_changed) mark_vertex(a, v, b, e, threadIdx.x, NT);
     }
 }
 
@@ -266,6 +309,10 @@ __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h
     uint32_t* C = s_dyn + cap_max;
     const int2 hc = h.chunks[blk];
     const int32_t v = h.hubs[hc.x];
+    if (!is_active<H>(a, v)) {  // block-uniform: the hub keeps its label
+        if (threadIdx.x == 0) h.hs.chunk_cnt[blk] = 0u;
+        return;
+    }
     const int32_t vb = a.off[v], ve = a.off[v + 1];
     const int32_t b = vb + hc.y * h.chunk;
     const int32_t e = min(b + h.chunk, ve);
@@ -359,6 +406,7 @@ __device__ __forceinline__ void hub_part_body(const StepArgs& a, const HubArgs& 
     __syncthreads();
     tot = 0u;
     for (int i = 0; i < kHubThreads / 64; ++i) tot += s_sum[i];
+    if (tot == 0u) return;  // block-uniform: inactive hub (frontier) or nothing to count
     const uint32_t nparts = max(1u, (tot + (uint32_t)h.part_keys - 1u) / (uint32_t)h.part_keys);
     if ((uint32_t)hp.y >= nparts) return;  // block-uniform
     const uint64_t span = ((1ull << 32) + nparts - 1) / nparts;
@@ -406,12 +454,31 @@ __device__ __forceinline__ void hub_final_body(const StepArgs& a, const HubArgs&
         const int32_t v = h.hubs[i];
         const unsigned long long best = h.hs.best[i];
         h.hs.best[i] = 0ull;
-        const int32_t old = a.keep[v];
-        const int32_t nl = packed_label(best);
-        H::update_vertex(a, v, nl, best);
-        changed = nl != old;
+        if (is_active<H>(a, v)) {
+            const int32_t old = a.keep[v];
+            const int32_t nl = packed_label(best);
+            H::update_vertex(a, v, nl, best);
+            changed = nl != old;
+        }
+        if (marking<H>(a)) {
+            h.hs.changed[i] = changed ? 1u : 0u;
+            if (changed) a.act_w[v] = 1;
+        }
     }
     count_changes(a, changed);
+}
+
+// Frontier: a changed hub marks its neighbours, one workgroup per chunk.
+template <class H>
+__device__ __forceinline__ void hub_mark_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
+    if (!marking<H>(a)) return;
+    const int2 hc = h.chunks[blk];
+    if (!h.hs.changed[hc.x]) return;
+    const int32_t v = h.hubs[hc.x];
+    const int32_t vb = a.off[v], ve = a.off[v + 1];
+    const int32_t b = vb + hc.y * h.chunk;
+    const int32_t e = min(b + h.chunk, ve);
+    for (int32_t j = b + (int32_t)threadIdx.x; j < e; j += 256) a.act_w[a.nbr[j]] = 1;
 }
 
 // ------------------------------------------------------------- wrappers ----
@@ -442,6 +509,10 @@ __global__ __launch_bounds__(kHubThreads) void k_hub_part(StepArgs a, HubArgs h)
 template <class H>
 __global__ __launch_bounds__(256) void k_hub_final(StepArgs a, HubArgs h) {
     hub_final_body<H>(a, h, blockIdx.x);
+}
+template <class H>
+__global__ __launch_bounds__(256) void k_hub_mark(StepArgs a, HubArgs h) {
+    hub_mark_body<H>(a, h, blockIdx.x);
 }
 
 // Folded jobs: cum[j] is the first workgroup of job j in this launch
@@ -587,9 +658,12 @@ __global__ void k_iota(int32_t* p, int64_t count) {
 }
 // Sums and clears the step's change counters into the ring and advances
 // the step counter.  One workgroup of kChangeSlots threads.
+// With a frontier control block fctl = {all_active, mark, threshold}: the
+// next step recomputes everything unless this step marked, and marks when
+// this step changed at most `threshold` labels.
 __global__ __launch_bounds__(kChangeSlots) void k_end_step(unsigned long long* acc,
                                                            unsigned long long* ring, int32_t* ctr,
-                                                           int32_t ring_size) {
+                                                           int32_t ring_size, int32_t* fctl) {
     __shared__ unsigned long long part[kChangeSlots / 64];
     unsigned long long x = acc[threadIdx.x];
     acc[threadIdx.x] = 0ull;
@@ -604,6 +678,11 @@ __global__ __launch_bounds__(kChangeSlots) void k_end_step(unsigned long long* a
         const int32_t t = *ctr;
         ring[t % ring_size] = sum;
         *ctr = t + 1;
+        if (fctl) {
+            const int32_t marked = fctl[1];
+            fctl[0] = marked ? 0 : 1;
+            fctl[1] = sum <= (unsigned long long)fctl[2] ? 1 : 0;
+        }
     }
 }
 
@@ -674,6 +753,8 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
         mark(GLP_TIER_AUX, 0);
         hipLaunchKernelGGL((k_hub_part<H>), dim3(np), dim3(kHubThreads), lds_part, s, a, h);
         hipLaunchKernelGGL((k_hub_final<H>), dim3(grid_for(h.nhubs, 256)), dim3(256), 0, s, a, h);
+        if (H::kFrontier && a.act_w)
+            hipLaunchKernelGGL((k_hub_mark<H>), dim3(nc), dim3(256), 0, s, a, h);
         mark(GLP_TIER_AUX, 1);
     } else {
         mark(GLP_TIER_HUB, 1);
@@ -792,8 +873,9 @@ int launch_iota(int32_t* p, int64_t count, hipStream_t s) {
 }
 
 int launch_end_step(unsigned long long* acc, unsigned long long* ring, int32_t* step_ctr,
-                    int32_t ring_size, hipStream_t s) {
-    hipLaunchKernelGGL(k_end_step, dim3(1), dim3(kChangeSlots), 0, s, acc, ring, step_ctr, ring_size);
+                    int32_t ring_size, hipStream_t s, int32_t* fctl) {
+    hipLaunchKernelGGL(k_end_step, dim3(1), dim3(kChangeSlots), 0, s, acc, ring, step_ctr, ring_size,
+                       fctl);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "end_step launch: %s", hipGetErrorString(e));
 }

@@ -22,6 +22,9 @@ struct glp_session {
     int32_t* adj = nullptr;              // pre-gathered neighbour labels (gather mode)
     int32_t* mem = nullptr;              // SLPA memory (T+1) x n
     void* cm_scratch = nullptr;          // counting-method scratch
+    uint8_t* act[2] = {nullptr, nullptr};  // frontier flags, double-buffered like L
+    int32_t* fctl = nullptr;             // frontier control {all_active, mark, threshold}
+    int64_t act_bytes = 0;
     glp::HubScratch hs;
     unsigned long long* d_acc = nullptr;   // change counter of the running step
     unsigned long long* d_ring = nullptr;  // change count of each finished step

@@ -127,10 +127,14 @@ static Method method_of(int32_t m) {
 
 int check_params(const HostCSR& g, const glp_params* p) {
     if (!p) return set_error(GLP_E_ARG, "null params");
-    for (int i = 0; i < 6; ++i)
+    for (int i = 0; i < 5; ++i)
         if (p->reserved[i]) return set_error(GLP_E_ARG, "reserved params fields must be zero");
     if (p->method < GLP_CLASSIC || p->method > GLP_BUFFER)
         return set_error(GLP_E_ARG, "unknown method %d", p->method);
+    if (p->frontier && p->method != GLP_CLASSIC)
+        return set_error(GLP_E_UNSUPPORTED,
+                         "the frontier is exact for classic LP only (layered volumes and SLPA speakers "
+                         "change every label's score each step)");
     if (p->weighted && !g.weighted()) return set_error(GLP_E_ARG, "weighted run on an unweighted graph");
     // frequencies are packed as non-negative int32 scores
     uint64_t wmax = 1;
@@ -181,6 +185,10 @@ glp_session::~glp_session() {
     (void)hipFree(mem);
     (void)hipFree(hs.pairs);
     (void)hipFree(hs.chunk_cnt);
+    (void)hipFree(hs.changed);
+    (void)hipFree(act[0]);
+    (void)hipFree(act[1]);
+    (void)hipFree(fctl);
     (void)hipFree(hs.best);
     (void)hipFree(d_acc);
     (void)hipFree(d_ring);
@@ -245,6 +253,13 @@ int session_create(glp_graph* g, const glp_params* prm, glp_session** out) {
         GLP_HIP(hipMalloc(&s->hs.chunk_cnt, p.hub_chunks.size() * 4));
         GLP_HIP(hipMalloc(&s->hs.best, p.hubs.size() * 8));
         GLP_HIP(hipMemsetAsync(s->hs.best, 0, p.hubs.size() * 8, s->stream));
+        if (prm->frontier) GLP_HIP(hipMalloc(&s->hs.changed, p.hubs.size() * 4));
+    }
+    if (prm->frontier) {
+        s->act_bytes = ((int64_t)nn + 4) & ~(int64_t)3;
+        GLP_HIP(hipMalloc(&s->act[0], (size_t)s->act_bytes));
+        GLP_HIP(hipMalloc(&s->act[1], (size_t)s->act_bytes));
+        GLP_HIP(hipMalloc(&s->fctl, 4 * sizeof(int32_t)));
     }
     GLP_HIP(hipMalloc(&s->d_acc, 8 * kChangeSlots));
     GLP_HIP(hipMalloc(&s->d_ring, 8 * kChangeRing));
@@ -282,6 +297,14 @@ int session_reset(glp_session* s) {
         if ((rc = launch_fill_i32(s->V[1], 0, n, s->stream))) return rc;
     }
     if (s->mem && (rc = launch_iota(s->mem, n, s->stream))) return rc;  // M_v[0] = v
+    if (s->fctl) {
+        // the first step recomputes every vertex and does not mark; marking
+        // starts after a step that changed at most n/32 labels
+        const int32_t ctl[4] = {1, 0, std::max<int32_t>(1, (int32_t)(n / 32)), 0};
+        GLP_HIP(hipMemcpyAsync(s->fctl, ctl, sizeof ctl, hipMemcpyHostToDevice, s->stream));
+        if ((rc = launch_zero_bytes(s->act[0], s->act_bytes, s->stream))) return rc;
+        if ((rc = launch_zero_bytes(s->act[1], s->act_bytes, s->stream))) return rc;
+    }
     GLP_HIP(hipMemsetAsync(s->d_acc, 0, 8 * kChangeSlots, s->stream));
     GLP_HIP(hipMemsetAsync(s->d_ring, 0, 8 * kChangeRing, s->stream));
     GLP_HIP(hipMemsetAsync(s->d_ctr, 0, 4, s->stream));
@@ -330,6 +353,11 @@ static StepArgs step_args(glp_session* s) {
             break;
     }
     a.adj = s->adj;
+    if (s->fctl) {
+        a.act_r = s->act[c];
+        a.act_w = s->act[c ^ 1];
+        a.fctl = s->fctl;
+    }
     return a;
 }
 
@@ -342,6 +370,8 @@ int enqueue_step(glp_session* s, hipEvent_t* ev) {
     int rc;
     if (s->method == Method::Layered && (rc = launch_fill_i32(s->V[s->cur ^ 1], 0, s->n, s->stream)))
         return rc;
+    // frontier flags for the next step start clear (they held the last step's)
+    if (s->fctl && (rc = launch_zero_bytes(s->act[s->cur ^ 1], s->act_bytes, s->stream))) return rc;
     if (ev) (void)hipEventRecord(ev[2 * GLP_TIER_PICK], s->stream);
     if (s->method == Method::Slpa || s->method == Method::User) {
         if ((rc = launch_pick(s->method, a, s->shown, s->stream))) return rc;
@@ -355,7 +385,7 @@ int enqueue_step(glp_session* s, hipEvent_t* ev) {
     if ((rc = launch_step(s->method, s->p.weighted != 0, s->p.gather != 0, *s->g->dev, s->hs, a,
                           s->stream, ev)))
         return rc;
-    if ((rc = launch_end_step(s->d_acc, s->d_ring, s->d_ctr, kChangeRing, s->stream))) return rc;
+    if ((rc = launch_end_step(s->d_acc, s->d_ring, s->d_ctr, kChangeRing, s->stream, s->fctl))) return rc;
     if (s->method != Method::Slpa) s->cur ^= 1;
     s->t += 1;
     return GLP_OK;

@@ -26,6 +26,7 @@ namespace glp {
 struct UserHooks {
     static constexpr bool kPick = true;
     static constexpr bool kSkipIsolated = false;  // update_vertex may have side effects
+    static constexpr bool kFrontier = false;      // scores may depend on more than neighbours
     __device__ static __forceinline__ int32_t pick_label(const StepArgs& a, int32_t vid) {
         return a.keep[vid];  // L[vid] = Lnext[vid]
     }

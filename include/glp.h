@@ -67,7 +67,11 @@ typedef struct glp_params {
     int32_t table_cap;    /* frequent / buffer table capacity C (1..64)          */
     int32_t gather;       /* 0: gather fused into counting; 1: separate gather   */
     int32_t profile;      /* 1: per-tier HIP-event timing inside each step       */
-    int32_t reserved[6];  /* must be zero                                        */
+    int32_t frontier;     /* classic only: 1 = recompute only vertices with a
+                             changed neighbour once few labels change (the
+                             reference's if_update frontier,
+                             header/kernel.cuh:21-35); labels are identical  */
+    int32_t reserved[5];  /* must be zero                                        */
 } glp_params;
 
 /* Per-tier timing and shape record.  Replaces header/test_result.h:5-23
@@ -198,6 +202,9 @@ int glp_batch_create(int32_t ngraphs, glp_graph* const* graphs,
 int glp_batch_run(glp_batch* b, int32_t steps, int32_t mode, double* ms);
 int glp_batch_labels(glp_batch* b, int32_t graph, int32_t* out);
 int glp_batch_reset(glp_batch* b);
+/* Autotunes tier thresholds and LDS sizes on the packed batch (glp_autotune)
+ * and resets the batch to L_0.                                               */
+int glp_batch_autotune(glp_batch* b, int32_t steps, double* best_ms);
 void glp_batch_free(glp_batch* b);
 
 /* ---- serving: continuous batching under an HBM budget ------------------ */

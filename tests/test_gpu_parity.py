@@ -164,6 +164,34 @@ def test_graph_replay_matches_eager_and_changes():
     assert ch.tolist() == [int((want[t] != prev[t]).sum()) for t in range(7)]
 
 
+@pytest.mark.parametrize("scale,tiny", [(14, False), (15, False), (13, True)])
+def test_frontier_equals_dense(scale, tiny):
+    """The frontier recomputes only vertices with a changed neighbour once a
+    step changes at most n/32 labels; every step must equal the dense run."""
+    kw = dict(pack_max_deg=8, wave_max_deg=16, block_max_deg=32, hub_chunk=256,
+              part_keys=256) if tiny else {}
+    with G.tuning(**kw):
+        g = G.Graph.rmat(scale, 16, seed=50 + scale)
+        off, nbr, _ = g.csr()
+        want = O.classic(off, nbr, 14)
+        _, dump, _ = G.run(g, 14, dump=True, frontier=True)
+        assert_steps_equal(dump, want)
+        with G.Session(g, frontier=True) as s:
+            s.step(2)
+            s.step(12, graph=True)
+            assert np.array_equal(s.labels(), want[-1])
+            ch = s.changes()
+    prev = np.vstack([np.arange(len(off) - 1, dtype=np.int32), want[:-1]])
+    assert ch.tolist() == [int((want[t] != prev[t]).sum()) for t in range(14)]
+    assert ch[-1] < (len(off) - 1) // 32   # the frontier was in use at the end
+
+
+def test_frontier_rejected_for_layered():
+    g = G.Graph.rmat(10, 8, seed=1)
+    with pytest.raises(G.GLPError):
+        G.run(g, 2, method=G.LAYERED, gamma=(1, 4), frontier=True)
+
+
 def test_checkpoint_resume():
     g = G.Graph.rmat(13, 16, seed=6)
     off, nbr, _ = g.csr()

@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_params():
     assert "gfx950" in G.version()
     p = G.make_params()
-    assert p.method == G.CLASSIC and p.gamma_den == 1 and list(p.reserved) == [0] * 6
+    assert p.method == G.CLASSIC and p.gamma_den == 1 and list(p.reserved) == [0] * 5
 
 
 def write_pairs(path, pairs, extra=b""):
