@@ -49,8 +49,10 @@ struct Plan {
     std::vector<int2> hub_chunks;             // (hub index, chunk index): aggregation workgroups
     std::vector<int32_t> hub_chunk0;          // index of each hub's first chunk
     std::vector<int64_t> hub_base;            // first pair slot of each hub (chunk regions)
-    std::vector<int2> hub_parts;              // (hub index, part index): counting workgroups
+    std::vector<int2> hub_parts;              // (hub index, bucket): counting workgroups
+    std::vector<int64_t> hub_hist_base;       // first histogram entry of each hub
     int64_t pair_slots = 0;                   // sum over hubs of chunks * hub_chunk
+    int64_t hist_slots = 0;                   // sum over hubs of buckets * chunks
     int64_t edges[4] = {}, verts[4] = {};     // per tier: packed, wave, block, hub
 };
 int build_plan(const HostCSR& g, const Tuning& t, Plan& p);
@@ -71,6 +73,7 @@ struct DeviceGraph {
     int32_t* d_hub_chunk0 = nullptr;
     int64_t* d_hub_base = nullptr;
     int2* d_hub_parts = nullptr;
+    int64_t* d_hub_hist_base = nullptr;
     double upload_ms = 0, plan_ms = 0;
     size_t bytes = 0;
     bool owns_csr = true;  // false: off/nbr/w borrowed from another DeviceGraph
@@ -88,11 +91,22 @@ size_t device_graph_bytes(const HostCSR& g, const Plan& p);
 // pairs, the number of pairs per chunk, and per-hub winners (zero between
 // steps).
 struct HubScratch {
-    unsigned long long* pairs = nullptr;
+    unsigned long long* pairs = nullptr;   // per chunk region: its distinct pairs
+    unsigned long long* sorted = nullptr;  // per hub: its pairs grouped by bucket
+    uint32_t* hist = nullptr;              // per hub: P x chunks counts, then offsets
+    uint32_t* total = nullptr;             // per hub: number of pairs
     uint32_t* chunk_cnt = nullptr;
     unsigned long long* best = nullptr;
     uint32_t* changed = nullptr;  // frontier: hub's label changed this step
 };
+
+// Buckets of a hub of degree d (host and device agree): enough that a bucket
+// expects at most part_keys distinct labels, at most kMaxBuckets.
+constexpr int kMaxBuckets = 4096;
+__host__ __device__ __forceinline__ uint32_t hub_buckets(int32_t d, int32_t part_keys) {
+    const uint32_t p = (uint32_t)(((int64_t)d + part_keys - 1) / part_keys);
+    return p < 1u ? 1u : (p > (uint32_t)kMaxBuckets ? (uint32_t)kMaxBuckets : p);
+}
 
 // Hub-tier launch arguments (plan lists plus session scratch).
 struct HubArgs {
@@ -101,6 +115,7 @@ struct HubArgs {
     const int2* parts;        // (hub, part) per counting workgroup
     const int32_t* chunk0;    // first chunk index of each hub
     const int64_t* base;      // first pair slot of each hub
+    const int64_t* hist_base; // first histogram entry of each hub
     int32_t chunk;            // edges per chunk = pair slots per chunk region
     int32_t part_keys;        // expected distinct labels per counting workgroup
     int32_t part_limit;       // distinct labels that trigger halving (0: table/2)
@@ -112,7 +127,8 @@ struct HubArgs {
 // tier work lists.  blocks[] holds the workgroups it needs in each launch,
 // in the order LAUNCH_* below.
 enum { kLaunchPacked = 0, kLaunchWave = 1, kLaunchBlock0 = 2,
-       kLaunchHubAgg = 2 + kMaxBlockClasses, kLaunchHubPart, kLaunchHubFinal, kLaunches };
+       kLaunchHubAgg = 2 + kMaxBlockClasses, kLaunchHubScan, kLaunchHubScatter, kLaunchHubPart,
+       kLaunchHubFinal, kLaunches };
 struct JobDesc {
     StepArgs a;
     const int2* groups;

@@ -283,37 +283,61 @@ __device__ __forceinline__ void block_body(const StepArgs& a, const int32_t* __r
 }
 
 // ------------------------------------------------------------------- hub ----
-// Hubs are counted in two passes and never touch an HBM hash table:
-//  k_hub_agg   one workgroup per chunk of `chunk` edges: gathers the labels,
-//              counts them in an LDS table and writes the chunk's distinct
-//              (label+1, frequency) pairs into the chunk's own pair region
-//  k_hub_part  one workgroup per (hub, part): owns the labels whose fmix32
-//              image falls in the part's share of the 32-bit range, streams
-//              the hub's pairs (L2-resident) and sums the matching ones in an
-//              LDS table.  The number of parts in use follows the pairs the
-//              chunks reported, so converged hubs need one part.  If a range
-//              holds more labels than the table, the workgroup counts the
-//              first half of the range, then the rest, halving again as
-//              needed; fmix32 is a bijection, so a range narrower than the
-//              table can never overflow and the loop always ends.
-//  k_hub_final one thread per hub: commits the winner.
+// Hubs are counted without any HBM hash table, in five passes:
+//  k_hub_agg     one workgroup per chunk of `chunk` edges: gathers the labels,
+//                counts them in an LDS table, writes the chunk's distinct
+//                (label+1, frequency) pairs to its pair region, and a
+//                histogram of those pairs over the hub's P buckets (a pair's
+//                bucket is fmix32(label+1) * P >> 32)
+//  k_hub_scan    one workgroup per hub: exclusive scan of the histogram in
+//                bucket-major order, i.e. where each chunk's share of each
+//                bucket starts once the hub's pairs are grouped by bucket
+//  k_hub_scatter one workgroup per chunk: moves its pairs into bucket order
+//  k_hub_part    one workgroup per (hub, bucket): sums its bucket's pairs in an
+//                LDS table and keeps the best; if the bucket holds more labels
+//                than the table, it counts the lower half of its fmix32 range,
+//                then the rest, halving as needed (fmix32 is a bijection, so a
+//                range no wider than the table cannot overflow)
+//  k_hub_final   one thread per hub: commits the winner
+// P = ceil(degree / part_keys), at most kMaxBuckets, so a bucket expects at
+// most part_keys distinct labels even when every neighbour shows a different
+// label; once labels converge most buckets are empty and exit at once.
 constexpr int kHubThreads = 256;
+
+// dynamic LDS of k_hub_agg: the chunk's table plus the bucket histogram
+static inline size_t hub_agg_lds(int32_t chunk) {
+    return (size_t)next_pow2(2u * (uint32_t)chunk) * 2 * sizeof(uint32_t) + kMaxBuckets * sizeof(uint32_t);
+}
+
+__device__ __forceinline__ uint32_t bucket_of(uint32_t hk, uint32_t P) {
+    return (uint32_t)(((uint64_t)hk * P) >> 32);
+}
+// first fmix32 value of bucket p: ceil(p * 2^32 / P)
+__device__ __forceinline__ uint64_t bucket_lo(uint32_t p, uint32_t P) {
+    return (((uint64_t)p << 32) + P - 1) / P;
+}
 
 template <class H, bool W, bool G>
 __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
-    // LDS table of next_pow2(2*chunk) slots (dynamic, sized at launch)
+    // dynamic LDS: table of next_pow2(2*chunk) keys and counts, then the
+    // bucket histogram (kMaxBuckets counters)
     extern __shared__ uint32_t s_dyn[];
     __shared__ uint32_t s_npairs;
     const uint32_t cap_max = next_pow2(2u * (uint32_t)h.chunk);
     uint32_t* K = s_dyn;
     uint32_t* C = s_dyn + cap_max;
+    uint32_t* s_hist = s_dyn + 2 * cap_max;
     const int2 hc = h.chunks[blk];
     const int32_t v = h.hubs[hc.x];
+    const int32_t vb = a.off[v], ve = a.off[v + 1];
+    const int32_t nch = (ve - vb + h.chunk - 1) / h.chunk;
+    const uint32_t P = hub_buckets(ve - vb, h.part_keys);
+    uint32_t* hist = h.hs.hist + h.hist_base[hc.x];  // [bucket][chunk]
     if (!is_active<H>(a, v)) {  // block-uniform: the hub keeps its label
+        for (uint32_t q = threadIdx.x; q < P; q += kHubThreads) hist[(int64_t)q * nch + hc.y] = 0u;
         if (threadIdx.x == 0) h.hs.chunk_cnt[blk] = 0u;
         return;
     }
-    const int32_t vb = a.off[v], ve = a.off[v + 1];
     const int32_t b = vb + hc.y * h.chunk;
     const int32_t e = min(b + h.chunk, ve);
     const uint32_t cap = min(max(next_pow2(2u * (uint32_t)(e - b)), 64u), cap_max);
@@ -322,6 +346,7 @@ __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h
         K[i] = 0u;
         C[i] = 0u;
     }
+    for (uint32_t q = threadIdx.x; q < P; q += kHubThreads) s_hist[q] = 0u;
     if (threadIdx.x == 0) s_npairs = 0u;
     __syncthreads();
     count_edges<H, W, G, 8, kHubThreads>(a, v, b, e, threadIdx.x, K, C, mask);
@@ -332,86 +357,131 @@ __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h
         if (!k) continue;
         const uint32_t slot = atomicAdd(&s_npairs, 1u);
         out[slot] = ((unsigned long long)k << 32) | C[i];
+        atomicAdd(&s_hist[bucket_of(fmix32(k), P)], 1u);
     }
     __syncthreads();
+    for (uint32_t q = threadIdx.x; q < P; q += kHubThreads) hist[(int64_t)q * nch + hc.y] = s_hist[q];
     if (threadIdx.x == 0) h.hs.chunk_cnt[blk] = s_npairs;
 }
 
-// Sums the (key, frequency) pairs whose fmix32 image lies in [lo, hi) into the
-// LDS table.  Returns false (and leaves the table partial) when more than
-// `limit` distinct keys arrive; the table has room for limit + kHubThreads.
-__device__ __forceinline__ bool part_pass(const unsigned long long* pairs, const uint32_t* cnt,
-                                          int32_t nch, int32_t chunk, uint64_t lo, uint64_t hi,
-                                          uint32_t* K, uint32_t* C, uint32_t shift,
-                                          uint32_t limit, uint32_t* s_distinct,
-                                          uint32_t* s_over) {
+// In-place exclusive scan of one hub's P x nch histogram (bucket-major);
+// also records the hub's pair total.
+__device__ __forceinline__ void hub_scan_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
+    __shared__ uint32_t s_part[kHubThreads];
+    const int32_t v = h.hubs[blk];
+    const int32_t d = a.off[v + 1] - a.off[v];
+    const int64_t nch = (d + h.chunk - 1) / h.chunk;
+    const int64_t total = (int64_t)hub_buckets(d, h.part_keys) * nch;
+    uint32_t* hist = h.hs.hist + h.hist_base[blk];
+    // each thread owns a contiguous run of entries
+    const int64_t per = (total + kHubThreads - 1) / kHubThreads;
+    const int64_t lo = min((int64_t)threadIdx.x * per, total);
+    const int64_t hi = min(lo + per, total);
+    uint32_t sum = 0u;
+    for (int64_t i = lo; i < hi; ++i) sum += hist[i];
+    s_part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0u;
+        for (int i = 0; i < kHubThreads; ++i) {
+            const uint32_t x = s_part[i];
+            s_part[i] = run;
+            run += x;
+        }
+        h.hs.total[blk] = run;
+    }
+    __syncthreads();
+    uint32_t run = s_part[threadIdx.x];
+    for (int64_t i = lo; i < hi; ++i) {
+        const uint32_t x = hist[i];
+        hist[i] = run;
+        run += x;
+    }
+}
+
+// Moves one chunk's pairs to their bucket positions in the hub's sorted area.
+__device__ __forceinline__ void hub_scatter_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
+    __shared__ uint32_t s_cur[kMaxBuckets];
+    const int2 hc = h.chunks[blk];
+    const uint32_t n = h.hs.chunk_cnt[blk];
+    if (n == 0u) return;  // block-uniform
+    const int32_t v = h.hubs[hc.x];
+    const int32_t d = a.off[v + 1] - a.off[v];
+    const int64_t nch = (d + h.chunk - 1) / h.chunk;
+    const uint32_t P = hub_buckets(d, h.part_keys);
+    const uint32_t* hist = h.hs.hist + h.hist_base[hc.x];
+    for (uint32_t q = threadIdx.x; q < P; q += kHubThreads) s_cur[q] = hist[(int64_t)q * nch + hc.y];
+    __syncthreads();
+    const unsigned long long* src = h.hs.pairs + h.base[hc.x] + (int64_t)hc.y * h.chunk;
+    unsigned long long* dst = h.hs.sorted + h.base[hc.x];
+    for (uint32_t i = threadIdx.x; i < n; i += kHubThreads) {
+        const unsigned long long pr = src[i];
+        const uint32_t q = bucket_of(fmix32((uint32_t)(pr >> 32)), P);
+        dst[atomicAdd(&s_cur[q], 1u)] = pr;
+    }
+}
+
+// Sums the pairs of pr[0..n) whose fmix32 image lies in [lo, hi) into the LDS
+// table.  Stops early (the caller halves the range) once more than `limit`
+// distinct keys arrived; the table has room for limit + kHubThreads.
+__device__ __forceinline__ void part_pass(const unsigned long long* pr, uint32_t n, uint64_t lo,
+                                          uint64_t hi, uint32_t* K, uint32_t* C, uint32_t shift,
+                                          uint32_t limit, uint32_t* s_distinct, uint32_t* s_over) {
     const uint32_t mask = (1u << (32 - shift)) - 1u;
-    for (int32_t c = 0; c < nch; ++c) {
-        const uint32_t nc = cnt[c];
-        const unsigned long long* pc = pairs + (int64_t)c * chunk;
-        for (uint32_t i = threadIdx.x; i < nc; i += kHubThreads) {
-            if (lds_load(s_over)) return false;
-            const unsigned long long pr = pc[i];
-            const uint32_t key = (uint32_t)(pr >> 32);
-            const uint32_t hk = fmix32(key);
-            if (hk < lo || hk >= hi) continue;
-            const uint32_t f = (uint32_t)pr;
-            uint32_t s = (hk * 0x9e3779b1u) >> shift;
-            for (;;) {
-                const uint32_t k = lds_load(&K[s]);
-                if (k == key) {
+    for (uint32_t i = threadIdx.x; i < n; i += kHubThreads) {
+        if (lds_load(s_over)) return;
+        const unsigned long long x = pr[i];
+        const uint32_t key = (uint32_t)(x >> 32);
+        const uint32_t hk = fmix32(key);
+        if (hk < lo || hk >= hi) continue;
+        const uint32_t f = (uint32_t)x;
+        uint32_t s = (hk * 0x9e3779b1u) >> shift;
+        for (;;) {
+            const uint32_t k = lds_load(&K[s]);
+            if (k == key) {
+                atomicAdd(&C[s], f);
+                break;
+            }
+            if (k == 0u) {
+                const uint32_t prev = atomicCAS(&K[s], 0u, key);
+                if (prev == 0u) {
+                    if (atomicAdd(s_distinct, 1u) + 1u > limit) atomicExch(s_over, 1u);
                     atomicAdd(&C[s], f);
                     break;
                 }
-                if (k == 0u) {
-                    const uint32_t prev = atomicCAS(&K[s], 0u, key);
-                    if (prev == 0u) {
-                        if (atomicAdd(s_distinct, 1u) + 1u > limit) atomicExch(s_over, 1u);
-                        atomicAdd(&C[s], f);
-                        break;
-                    }
-                    if (prev == key) {
-                        atomicAdd(&C[s], f);
-                        break;
-                    }
+                if (prev == key) {
+                    atomicAdd(&C[s], f);
+                    break;
                 }
-                s = (s + 1u) & mask;
             }
+            s = (s + 1u) & mask;
         }
     }
-    return true;
 }
 
 template <class H>
 __device__ __forceinline__ void hub_part_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
     extern __shared__ uint32_t s_dyn[];
     __shared__ unsigned long long s_red[kHubThreads / 64];
-    __shared__ uint32_t s_sum[kHubThreads / 64];
     __shared__ uint32_t s_distinct, s_over;
+    const int2 hp = h.parts[blk];
+    const int32_t v = h.hubs[hp.x];
+    const int32_t d = a.off[v + 1] - a.off[v];
+    const int64_t nch = (d + h.chunk - 1) / h.chunk;
+    const uint32_t P = hub_buckets(d, h.part_keys);
+    const uint32_t q = (uint32_t)hp.y;
+    const uint32_t* hist = h.hs.hist + h.hist_base[hp.x];
+    const uint32_t start = hist[(int64_t)q * nch];
+    const uint32_t stop = q + 1 < P ? hist[(int64_t)(q + 1) * nch] : h.hs.total[hp.x];
+    if (start == stop) return;  // block-uniform: empty bucket (or inactive hub)
+    const unsigned long long* pr = h.hs.sorted + h.base[hp.x] + start;
+    const uint32_t n = stop - start;
     const uint32_t cap = next_pow2(4u * (uint32_t)h.part_keys);
     const uint32_t shift = 32u - (31u - __builtin_clz(cap));
     uint32_t* K = s_dyn;
     uint32_t* C = s_dyn + cap;
-    const int2 hp = h.parts[blk];
-    const int32_t v = h.hubs[hp.x];
-    const int32_t d = a.off[v + 1] - a.off[v];
-    const int32_t nch = (d + h.chunk - 1) / h.chunk;
-    const uint32_t* cnt = h.hs.chunk_cnt + h.chunk0[hp.x];
-    const unsigned long long* pairs = h.hs.pairs + h.base[hp.x];
-    // parts in use: enough for the distinct pairs the chunks reported
-    uint32_t tot = 0u;
-    for (int32_t c = threadIdx.x; c < nch; c += kHubThreads) tot += cnt[c];
-    tot = wave_sum_u32(tot);
-    if (lane_id() == 0) s_sum[threadIdx.x >> 6] = tot;
-    __syncthreads();
-    tot = 0u;
-    for (int i = 0; i < kHubThreads / 64; ++i) tot += s_sum[i];
-    if (tot == 0u) return;  // block-uniform: inactive hub (frontier) or nothing to count
-    const uint32_t nparts = max(1u, (tot + (uint32_t)h.part_keys - 1u) / (uint32_t)h.part_keys);
-    if ((uint32_t)hp.y >= nparts) return;  // block-uniform
-    const uint64_t span = ((1ull << 32) + nparts - 1) / nparts;
-    const uint64_t end = min((uint64_t)(hp.y + 1) * span, 1ull << 32);
-    uint64_t lo = (uint64_t)hp.y * span, hi = end;
+    const uint64_t end = bucket_lo(q + 1, P);
+    uint64_t lo = bucket_lo(q, P), hi = end;
     const uint32_t limit = h.part_limit > 0 ? (uint32_t)h.part_limit : cap / 2u;
     uint64_t best = 0ull;
     while (lo < end) {
@@ -424,7 +494,7 @@ __device__ __forceinline__ void hub_part_body(const StepArgs& a, const HubArgs& 
             s_over = 0u;
         }
         __syncthreads();
-        part_pass(pairs, cnt, nch, h.chunk, lo, hi, K, C, shift, limit, &s_distinct, &s_over);
+        part_pass(pr, n, lo, hi, K, C, shift, limit, &s_distinct, &s_over);
         __syncthreads();
         const bool over = s_over != 0u;
         __syncthreads();
@@ -502,6 +572,12 @@ template <class H, bool W, bool G>
 __global__ __launch_bounds__(kHubThreads) void k_hub_agg(StepArgs a, HubArgs h) {
     hub_agg_body<H, W, G>(a, h, blockIdx.x);
 }
+__global__ __launch_bounds__(kHubThreads) void k_hub_scan(StepArgs a, HubArgs h) {
+    hub_scan_body(a, h, blockIdx.x);
+}
+__global__ __launch_bounds__(kHubThreads) void k_hub_scatter(StepArgs a, HubArgs h) {
+    hub_scatter_body(a, h, blockIdx.x);
+}
 template <class H>
 __global__ __launch_bounds__(kHubThreads) void k_hub_part(StepArgs a, HubArgs h) {
     hub_part_body<H>(a, h, blockIdx.x);
@@ -543,6 +619,10 @@ __global__ __launch_bounds__(NT) void k_multi(const JobDesc* __restrict__ jobs,
         block_body<H, W, G, NT>(jd.a, jd.block[L - kLaunchBlock0], cap_max, blk);
     } else if constexpr (L == kLaunchHubAgg) {
         hub_agg_body<H, W, G>(jd.a, jd.hub, blk);
+    } else if constexpr (L == kLaunchHubScan) {
+        hub_scan_body(jd.a, jd.hub, blk);
+    } else if constexpr (L == kLaunchHubScatter) {
+        hub_scatter_body(jd.a, jd.hub, blk);
     } else if constexpr (L == kLaunchHubPart) {
         hub_part_body<H>(jd.a, jd.hub, blk);
     } else {
@@ -583,10 +663,17 @@ static int launch_multi_t(const JobDesc* jobs, const int32_t* cum, int32_t nj, c
     launch_multi_block<H, W, 3>(jobs, cum, nj, grid, t, (uint32_t)class_cap[3], s);
     launch_multi_block<H, W, 4>(jobs, cum, nj, grid, t, (uint32_t)class_cap[4], s);
     if (grid[kLaunchHubAgg]) {
-        const size_t lds = (size_t)next_pow2(2u * (uint32_t)t.hub_chunk) * 2 * sizeof(uint32_t);
+        const size_t lds = hub_agg_lds(t.hub_chunk);
         hipLaunchKernelGGL((k_multi<H, W, false, kLaunchHubAgg, kHubThreads>), dim3(grid[kLaunchHubAgg]),
                            dim3(kHubThreads), lds, s, jobs, row(kLaunchHubAgg), nj, 0u);
     }
+    if (grid[kLaunchHubScan])
+        hipLaunchKernelGGL((k_multi<H, W, false, kLaunchHubScan, kHubThreads>), dim3(grid[kLaunchHubScan]),
+                           dim3(kHubThreads), 0, s, jobs, row(kLaunchHubScan), nj, 0u);
+    if (grid[kLaunchHubScatter])
+        hipLaunchKernelGGL((k_multi<H, W, false, kLaunchHubScatter, kHubThreads>),
+                           dim3(grid[kLaunchHubScatter]), dim3(kHubThreads), 0, s, jobs,
+                           row(kLaunchHubScatter), nj, 0u);
     if (grid[kLaunchHubPart]) {
         const size_t lds = (size_t)next_pow2(4u * (uint32_t)t.part_keys) * 2 * sizeof(uint32_t);
         hipLaunchKernelGGL((k_multi<H, W, false, kLaunchHubPart, kHubThreads>), dim3(grid[kLaunchHubPart]),
@@ -612,6 +699,8 @@ void job_blocks(const Plan& p, int32_t* blocks) {
     blocks[kLaunchWave] = (int32_t)((p.wave.size() + 3) / 4);
     for (int c = 0; c < p.nclasses; ++c) blocks[kLaunchBlock0 + c] = (int32_t)p.block[c].size();
     blocks[kLaunchHubAgg] = (int32_t)p.hub_chunks.size();
+    blocks[kLaunchHubScan] = (int32_t)p.hubs.size();
+    blocks[kLaunchHubScatter] = (int32_t)p.hub_chunks.size();
     blocks[kLaunchHubPart] = (int32_t)p.hub_parts.size();
     blocks[kLaunchHubFinal] = (int32_t)((p.hubs.size() + 255) / 256);
 }
@@ -739,6 +828,7 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
         h.parts = g.d_hub_parts;
         h.chunk0 = g.d_hub_chunk0;
         h.base = g.d_hub_base;
+        h.hist_base = g.d_hub_hist_base;
         h.chunk = p.tune.hub_chunk;
         h.part_keys = p.tune.part_keys;
         h.part_limit = p.tune.part_limit;
@@ -746,11 +836,13 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
         h.hs = hs;
         const unsigned nc = (unsigned)p.hub_chunks.size();
         const unsigned np = (unsigned)p.hub_parts.size();
-        const size_t lds_agg = (size_t)next_pow2(2u * (uint32_t)h.chunk) * 2 * sizeof(uint32_t);
         const size_t lds_part = (size_t)next_pow2(4u * (uint32_t)h.part_keys) * 2 * sizeof(uint32_t);
-        hipLaunchKernelGGL((k_hub_agg<H, W, G>), dim3(nc), dim3(kHubThreads), lds_agg, s, a, h);
+        hipLaunchKernelGGL((k_hub_agg<H, W, G>), dim3(nc), dim3(kHubThreads), hub_agg_lds(h.chunk), s, a,
+                           h);
         mark(GLP_TIER_HUB, 1);
         mark(GLP_TIER_AUX, 0);
+        hipLaunchKernelGGL(k_hub_scan, dim3(h.nhubs), dim3(kHubThreads), 0, s, a, h);
+        hipLaunchKernelGGL(k_hub_scatter, dim3(nc), dim3(kHubThreads), 0, s, a, h);
         hipLaunchKernelGGL((k_hub_part<H>), dim3(np), dim3(kHubThreads), lds_part, s, a, h);
         hipLaunchKernelGGL((k_hub_final<H>), dim3(grid_for(h.nhubs, 256)), dim3(256), 0, s, a, h);
         if (H::kFrontier && a.act_w)

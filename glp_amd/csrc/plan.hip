@@ -151,10 +151,11 @@ int build_plan(const HostCSR& g, const Tuning& t, Plan& p) {
             hub_chunks += nch;
             p.hub_base.push_back(p.pair_slots);
             p.pair_slots += (int64_t)nch * t.hub_chunk;
-            // enough counting workgroups for d distinct labels; the kernel uses
-            // fewer when the chunks report fewer
-            const int32_t nparts = (d + t.part_keys - 1) / t.part_keys;
-            for (int32_t q = 0; q < nparts; ++q) p.hub_parts.push_back(make_int2(h, q));
+            // one counting workgroup per bucket; empty buckets exit at once
+            const int32_t nb = (int32_t)hub_buckets(d, t.part_keys);
+            for (int32_t q = 0; q < nb; ++q) p.hub_parts.push_back(make_int2(h, q));
+            p.hub_hist_base.push_back(p.hist_slots);
+            p.hist_slots += (int64_t)nb * nch;
             p.edges[3] += d;
             p.verts[3]++;
         }

@@ -57,6 +57,9 @@ struct ServerJob {
             L[i] = V[i] = nullptr;
         }
         (void)hipFree(hs.pairs);
+        (void)hipFree(hs.sorted);
+        (void)hipFree(hs.hist);
+        (void)hipFree(hs.total);
         (void)hipFree(hs.chunk_cnt);
         (void)hipFree(hs.best);
         (void)hipFree(changes);
@@ -105,7 +108,8 @@ static size_t job_bytes(const HostCSR& g, const Plan& plan, const glp_params& p)
     const size_t n = std::max<size_t>((size_t)g.n, 1);
     b += 2 * n * 4;                                  // labels
     if (p.method == GLP_LAYERED) b += 2 * n * 4;     // volumes
-    b += (size_t)plan.pair_slots * 8 + plan.hub_chunks.size() * 4 + plan.hubs.size() * 8;
+    b += (size_t)plan.pair_slots * 16 + (size_t)plan.hist_slots * 4 + plan.hub_chunks.size() * 4 +
+         plan.hubs.size() * 12;
     b += 8 * kChangeSlots;
     return b;
 }
@@ -127,6 +131,9 @@ int glp_server::admit(ServerJob* j) {
     }
     if (!p.hubs.empty()) {
         GLP_HIP(hipMalloc(&j->hs.pairs, (size_t)p.pair_slots * 8));
+        GLP_HIP(hipMalloc(&j->hs.sorted, (size_t)p.pair_slots * 8));
+        GLP_HIP(hipMalloc(&j->hs.hist, (size_t)p.hist_slots * 4));
+        GLP_HIP(hipMalloc(&j->hs.total, p.hubs.size() * 4));
         GLP_HIP(hipMalloc(&j->hs.chunk_cnt, p.hub_chunks.size() * 4));
         GLP_HIP(hipMalloc(&j->hs.best, p.hubs.size() * 8));
         GLP_HIP(hipMemsetAsync(j->hs.best, 0, p.hubs.size() * 8, stream));
@@ -186,6 +193,7 @@ int glp_server::tick() {
         h.parts = dg.d_hub_parts;
         h.chunk0 = dg.d_hub_chunk0;
         h.base = dg.d_hub_base;
+        h.hist_base = dg.d_hub_hist_base;
         h.chunk = tune.hub_chunk;
         h.part_keys = tune.part_keys;
         h.part_limit = tune.part_limit;

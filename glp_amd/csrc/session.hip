@@ -35,6 +35,7 @@ DeviceGraph::~DeviceGraph() {
     (void)hipFree(d_hub_chunk0);
     (void)hipFree(d_hub_base);
     (void)hipFree(d_hub_parts);
+    (void)hipFree(d_hub_hist_base);
 }
 
 template <class T>
@@ -101,6 +102,8 @@ int device_plan_upload(DeviceGraph* dg, hipStream_t s) {
     if ((rc = upload(&dg->d_hub_chunk0, p.hub_chunk0.data(), p.hub_chunk0.size(), s, bytes))) return rc;
     if ((rc = upload(&dg->d_hub_base, p.hub_base.data(), p.hub_base.size(), s, bytes))) return rc;
     if ((rc = upload(&dg->d_hub_parts, p.hub_parts.data(), p.hub_parts.size(), s, bytes))) return rc;
+    if ((rc = upload(&dg->d_hub_hist_base, p.hub_hist_base.data(), p.hub_hist_base.size(), s, bytes)))
+        return rc;
     GLP_HIP(hipStreamSynchronize(s));
     dg->bytes += bytes;
     return GLP_OK;
@@ -184,6 +187,9 @@ glp_session::~glp_session() {
     (void)hipFree(adj);
     (void)hipFree(mem);
     (void)hipFree(hs.pairs);
+    (void)hipFree(hs.sorted);
+    (void)hipFree(hs.hist);
+    (void)hipFree(hs.total);
     (void)hipFree(hs.chunk_cnt);
     (void)hipFree(hs.changed);
     (void)hipFree(act[0]);
@@ -211,7 +217,8 @@ size_t session_bytes(const HostCSR& g, const Plan& p, const glp_params& prm) {
     if (prm.method == GLP_SLPA) b += n * 4 + (size_t)(prm.slpa_T + 1) * n * 4;
     if (prm.method == GLP_USER) b += n * 4;
     if (prm.gather) b += (size_t)g.m * 4;
-    b += (size_t)p.pair_slots * 8 + p.hub_chunks.size() * 4 + p.hubs.size() * 8;
+    b += (size_t)p.pair_slots * 16 + (size_t)p.hist_slots * 4 + p.hub_chunks.size() * 4 +
+         p.hubs.size() * 12;
     b += 8 * kChangeSlots + 8 * kChangeRing + 4;
     return b;
 }
@@ -250,6 +257,9 @@ int session_create(glp_graph* g, const glp_params* prm, glp_session** out) {
     if (prm->gather) GLP_HIP(hipMalloc(&s->adj, std::max<size_t>((size_t)g->csr.m, 1) * 4));
     if (!p.hubs.empty()) {
         GLP_HIP(hipMalloc(&s->hs.pairs, (size_t)p.pair_slots * 8));
+        GLP_HIP(hipMalloc(&s->hs.sorted, (size_t)p.pair_slots * 8));
+        GLP_HIP(hipMalloc(&s->hs.hist, (size_t)p.hist_slots * 4));
+        GLP_HIP(hipMalloc(&s->hs.total, p.hubs.size() * 4));
         GLP_HIP(hipMalloc(&s->hs.chunk_cnt, p.hub_chunks.size() * 4));
         GLP_HIP(hipMalloc(&s->hs.best, p.hubs.size() * 8));
         GLP_HIP(hipMemsetAsync(s->hs.best, 0, p.hubs.size() * 8, s->stream));
