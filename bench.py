@@ -21,7 +21,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from glp_amd import rocprof_summary as RS  # noqa: E402  (reads committed rocprof summaries only)
+
 HBM_PEAK_GBS = 8000.0
+# calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE per step of each kernel family
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_rmat_s25_classic.txt")
 METRIC = "GTEPS per LP iteration (RMAT-25, 1 GPU); batched graphs/s vs plain; p50/p99"
 
 
@@ -264,14 +268,19 @@ def main():
         tiers[name] = {"ms": ms, "edges": tim["tier_edges"].get(name, 0),
                        "vertices": tim["tier_vertices"].get(name, 0)}
     kernel_names = {"packed": "k_packed", "wave": "k_wave", "block": "k_block (all degree classes)",
-                    "hub": "k_hub_count", "aux": "k_hub_reduce+k_hub_final"}
-    dom = max((k for k in tiers if k in kernel_names), key=lambda k: tiers[k]["ms"], default=None)
+                    "hub": "k_hub_agg", "aux": "k_hub_scan+k_hub_scatter+k_hub_part+k_hub_final"}
+    families = {"packed": ["k_packed"], "wave": ["k_wave"], "block": ["k_block"],
+                "hub": ["k_hub_agg"]}
+    dom = max((k for k in tiers if k in families), key=lambda k: tiers[k]["ms"], default=None)
     roofline = None
-    if dom is not None and dom != "aux":
+    wl_key = "rmat-s%d-ef%d-seed%d-classic" % (args.scale, args.edge_factor, args.seed)
+    if dom is not None:
         b = tier_bytes(tiers[dom]["edges"], tiers[dom]["vertices"])
         achieved = b / (tiers[dom]["ms"] / 1e3) / 1e9
+        traffic = RS.traffic_for(TRAFFIC_FILE, wl_key, "main", families[dom])
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": None if traffic is None else int(traffic),
                     "kernel": kernel_names[dom], "kernel_ms": round(tiers[dom]["ms"], 4),
                     "algorithmic_bytes": int(b)}
     whole = tier_bytes(m, n)
@@ -292,7 +301,9 @@ def main():
         g1 = gs.timing()
         n_l = g1["tier_launches"]["gather"] - g0["tier_launches"]["gather"]
         gms = (g1["tier_ms"]["gather"] - g0["tier_ms"]["gather"]) / max(n_l, 1)
+        gt = RS.traffic_for(TRAFFIC_FILE, wl_key, "gather", ["k_gather"])
         gather = {"kernel": "k_gather", "kernel_ms": round(gms, 4), "algorithmic_bytes": 12 * m,
+                  "traffic": None if gt is None else int(gt),
                   "achieved_GBs": round(12 * m / (gms / 1e3) / 1e9, 1),
                   "frac": round(12 * m / (gms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                   "step_ms": round((g1["steps_ms"] - g0["steps_ms"]) / args.gather_steps, 4)}

@@ -25,6 +25,14 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t x, int src) {
 }
 __device__ __forceinline__ uint64_t max_u64(uint64_t a, uint64_t b) { return a > b ? a : b; }
 
+// Load of data read once per step (CSR neighbours, weights, gathered
+// labels): marked nontemporal so it does not evict the label lines that the
+// random gathers keep reusing from L2.
+template <class T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+
 // Max over the 64 lanes; every lane receives the result.
 __device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
     const int lane = lane_id();

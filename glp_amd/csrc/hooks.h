@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "device_prims.h"
+
 namespace glp {
 
 // Pointers and scalars one step needs; passed by value to every kernel.
@@ -75,8 +77,8 @@ struct ClassicHooks {
                                                              int32_t j, uint32_t* freq) {
         (void)vid;
         // a weighted launch may fold in unweighted jobs (w == nullptr)
-        *freq = W ? (a.w ? a.w[j] : 1u) : 1u;
-        return (uint32_t)(G ? a.adj[j] : a.L[a.nbr[j]]) + 1u;
+        *freq = W ? (a.w ? dev::ld_stream(&a.w[j]) : 1u) : 1u;
+        return (uint32_t)(G ? dev::ld_stream(&a.adj[j]) : a.L[dev::ld_stream(&a.nbr[j])]) + 1u;
     }
     __device__ static __forceinline__ uint64_t label_score(const StepArgs& a, int32_t vid,
                                                            uint32_t key, uint32_t freq) {

@@ -13,10 +13,11 @@
 //   block   deg <= block_max  one workgroup per vertex, LDS hash table sized
 //                             by the vertex's degree class
 //   hub     deg >  block_max  several workgroups per vertex: each counts a
-//                             chunk in LDS and flushes (label,count) pairs
-//                             into the vertex's HBM hash table; a second pass
-//                             walks the occupied slots, takes the maximum and
-//                             clears them, so no table memset is ever needed
+//                             chunk in LDS and writes its distinct
+//                             (label,count) pairs; a counting sort groups the
+//                             hub's pairs by label-hash bucket and one
+//                             workgroup per bucket sums them in LDS, so no
+//                             HBM hash table (and no memset) is needed
 // Counts are exact integer sums and the winner is a maximum over packed
 // (score,label) words, so every tier computes the same function and the
 // result does not depend on scheduling.
@@ -57,27 +58,58 @@ __device__ __forceinline__ void lds_insert(uint32_t* K, uint32_t* C, uint32_t ma
     }
 }
 
-// Adds this wave's changed-label count to one of kChangeSlots counters
-// (chosen by workgroup), so no single address takes every wave's atomic.
+// Inserts one (key, f) per lane (key 0: nothing) into an LDS table; all 64
+// lanes of the wave must be active.  Once labels converge most of a vertex's
+// edges show the same label and 64 atomics on one LDS address serialise, so
+// up to kPeels times the lanes holding the first pending lane's key are
+// merged into one insert of their summed frequency (sums wrap mod 2^32 as
+// the table's atomics do, so the counts are unchanged).
+constexpr int kPeels = 2;
+template <bool W>
+__device__ __forceinline__ void wave_insert(uint32_t* Kt, uint32_t* Ct, uint32_t mask, uint32_t key,
+                                            uint32_t f) {
+    const int lane = lane_id();
+    uint32_t ik = key, iff = f;  // what this lane finally inserts
+#pragma unroll
+    for (int r = 0; r < kPeels; ++r) {
+        const unsigned long long rest = __ballot(key != 0u);
+        if (rest == 0ull) break;
+        const int first = (int)__builtin_ctzll(rest);
+        const uint32_t lead = (uint32_t)__builtin_amdgcn_readlane((int)key, first);
+        const bool same = key == lead;
+        const unsigned long long sm = __ballot(same);
+        if (__popcll(sm) < 2) break;  // a singleton gains nothing
+        const uint32_t tot = W ? wave_sum_u32(same ? f : 0u) : (uint32_t)__popcll(sm);
+        if (same) {
+            ik = lane == first ? lead : 0u;
+            iff = tot;
+            key = 0u;
+        }
+    }
+    if (ik) lds_insert(Kt, Ct, mask, ik, iff);
+}
+
 // Counts edges [b, e) of vertex v into an LDS table, STRIDE threads
-// cooperating (thread `tid`).  Each thread first issues the loads of K edges
-// (neighbour ids, then their labels) and only then inserts them, so K
-// gathers are in flight per thread instead of one.
+// cooperating (thread `tid`; STRIDE a multiple of 64, whole waves).  Each
+// thread first issues the loads of K edges (neighbour ids, then their
+// labels) and only then inserts them, so K gathers are in flight per thread
+// instead of one.  The loop is wave-uniform (wave_insert needs every lane).
 template <class H, bool W, bool G, int K, int STRIDE>
 __device__ __forceinline__ void count_edges(const StepArgs& a, int32_t v, int32_t b, int32_t e,
                                             int tid, uint32_t* Kt, uint32_t* Ct, uint32_t mask) {
-    for (int32_t j0 = b + tid; j0 < e; j0 += STRIDE * K) {
+    static_assert(STRIDE % 64 == 0, "whole waves");
+    const int lane = tid & 63;
+    for (int32_t jw = b + (tid & ~63); jw < e; jw += STRIDE * K) {
         uint32_t key[K], fr[K];
 #pragma unroll
         for (int i = 0; i < K; ++i) {
-            const int32_t j = j0 + i * STRIDE;
+            const int32_t j = jw + lane + i * STRIDE;
             key[i] = 0u;
             fr[i] = 0u;
             if (j < e) key[i] = H::template load_neighbor<W, G>(a, v, j, &fr[i]);
         }
 #pragma unroll
-        for (int i = 0; i < K; ++i)
-            if (key[i]) lds_insert(Kt, Ct, mask, key[i], fr[i]);
+        for (int i = 0; i < K; ++i) wave_insert<W>(Kt, Ct, mask, key[i], fr[i]);
     }
 }
 
@@ -352,10 +384,18 @@ __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h
     count_edges<H, W, G, 8, kHubThreads>(a, v, b, e, threadIdx.x, K, C, mask);
     __syncthreads();
     unsigned long long* out = h.hs.pairs + h.base[hc.x] + (int64_t)hc.y * h.chunk;
+    // cap is a power of two >= 64, so each wave runs this loop whole; one
+    // LDS atomic per wave reserves its pair slots
     for (uint32_t i = threadIdx.x; i < cap; i += kHubThreads) {
         const uint32_t k = K[i];
+        const unsigned long long bal = __ballot(k != 0u);
+        if (bal == 0ull) continue;
+        uint32_t first = 0u;
+        if (lane_id() == 0) first = atomicAdd(&s_npairs, (uint32_t)__popcll(bal));
+        first = (uint32_t)__builtin_amdgcn_readfirstlane((int)first);
         if (!k) continue;
-        const uint32_t slot = atomicAdd(&s_npairs, 1u);
+        const uint32_t slot = first + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
         out[slot] = ((unsigned long long)k << 32) | C[i];
         atomicAdd(&s_hist[bucket_of(fmix32(k), P)], 1u);
     }
