@@ -79,7 +79,9 @@ def bench_batched(args, G):
     log("mix: %d graphs, %d vertices, %d edges" % (len(graphs), n_tot, m_tot))
     b = G.Batch(graphs, method=G.CLASSIC)
     tuned_ms = b.autotune(3) if not args.no_autotune else None
-    log("autotuned" if tuned_ms else "not autotuned")
+    chosen = {k: G.tuning_get(k) for k in ("pack_max_deg", "wave_max_deg", "block_max_deg",
+                                           "block_threads", "hub_chunk", "part_keys")}
+    log(("autotuned: %s" % chosen) if tuned_ms else "not autotuned")
     steps = args.steps
     # plain: one session per graph, stepped one graph after another
     b.run(1, mode=2)
@@ -104,7 +106,7 @@ def bench_batched(args, G):
         "vs_baseline": None, "dtype": "int32", "data": "synthetic",
         "config": {"workload": "batched classic LP over %d planted-partition graphs" % len(graphs),
                    "graphs": len(graphs), "n_total": n_tot, "m_total": m_tot, "seed": args.seed,
-                   "autotuned": tuned_ms is not None},
+                   "autotuned": tuned_ms is not None, "tuning": chosen},
         "plain_graphs_per_s": round(plain_gps, 2), "speedup_vs_plain": round(gps / plain_gps, 3),
         "batched_eager_ms": round(eager_ms, 2), "plain_ms": round(plain_ms, 2),
         "batched_ms": round(batch_ms, 2), "labels_identical": bool(same),
@@ -203,6 +205,11 @@ def main():
     ap.add_argument("--graph-replay", action="store_true", help="replay steps from a HIP graph")
     ap.add_argument("--gather-steps", type=int, default=3,
                     help="steps of the separate-gather mode timed after the main run (0: skip)")
+    ap.add_argument("--frontier", type=int, default=1,
+                    help="1: also time the exact-frontier mode on the same steps (reported apart)")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="tuning override (pack_max_deg, wave_max_deg, block_max_deg, hub_chunk, "
+                         "part_keys, part_limit, block_threads); repeatable")
     ap.add_argument("--workload", choices=["rmat", "batched", "served", "copy"], default="rmat")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--graphs", type=int, default=1024)
@@ -218,6 +225,11 @@ def main():
         sys.exit(2)
 
     import glp_amd as G
+    tuned = {}
+    for kv in args.tune:
+        k, v = kv.split("=", 1)
+        G.tuning_set(k, int(v))
+        tuned[k] = int(v)
     if G.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
@@ -283,11 +295,48 @@ def main():
                     "traffic": None if traffic is None else int(traffic),
                     "kernel": kernel_names[dom], "kernel_ms": round(tiers[dom]["ms"], 4),
                     "algorithmic_bytes": int(b)}
+        if traffic is not None:
+            # measured HBM bytes (rocprof, calibrated) over this run's duration
+            tg = traffic / (tiers[dom]["ms"] / 1e3) / 1e9
+            roofline["traffic_GBs"] = round(tg, 1)
+            roofline["traffic_frac"] = round(tg / HBM_PEAK_GBS, 4)
     whole = tier_bytes(m, n)
     step_roofline = {"algorithmic_bytes": int(whole),
                      "achieved_GBs": round(whole / (ms_per_step / 1e3) / 1e9, 1),
                      "frac": round(whole / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+    dense_labels = sess.labels()
     sess.free()
+
+    # the exact frontier (classic only): same steps, vertices whose neighbours
+    # all kept their labels are skipped; labels must equal the dense run's
+    frontier = None
+    if args.frontier:
+        fs = G.Session(g, G.make_params(method=G.CLASSIC, frontier=True))
+        fs.step(args.warmup)
+        fs.sync()
+        t0 = time.perf_counter()
+        fs.step(args.steps)
+        fs.sync()
+        fms = (time.perf_counter() - t0) * 1e3 / args.steps
+        same = bool(np.array_equal(fs.labels(), dense_labels))
+        frontier = {"ms_per_step": round(fms, 4), "edges_per_s_equiv_G": round(m / (fms / 1e3) / 1e9, 2),
+                    "labels_identical_to_dense": same}
+        log("frontier %.3f ms/step, labels identical: %s" % (fms, same))
+        fs.free()
+        # the separate gather with the frontier: only edges whose source
+        # changed in the previous step are re-gathered once few labels change
+        fg = G.Session(g, G.make_params(method=G.CLASSIC, frontier=True, gather=True))
+        fg.step(args.warmup)
+        fg.sync()
+        t0 = time.perf_counter()
+        fg.step(args.steps)
+        fg.sync()
+        gms_f = (time.perf_counter() - t0) * 1e3 / args.steps
+        same_g = bool(np.array_equal(fg.labels(), dense_labels))
+        frontier["gather_ms_per_step"] = round(gms_f, 4)
+        frontier["gather_labels_identical_to_dense"] = same_g
+        log("frontier gather %.3f ms/step, labels identical: %s" % (gms_f, same_g))
+        fg.free()
 
     # the separate gather pass (LoadNeighbor as its own kernel): 12 B per edge
     gather = None
@@ -307,6 +356,9 @@ def main():
                   "achieved_GBs": round(12 * m / (gms / 1e3) / 1e9, 1),
                   "frac": round(12 * m / (gms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                   "step_ms": round((g1["steps_ms"] - g0["steps_ms"]) / args.gather_steps, 4)}
+        if gt is not None:
+            gather["traffic_GBs"] = round(gt / (gms / 1e3) / 1e9, 1)
+            gather["traffic_frac"] = round(gt / (gms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
         log("gather kernel %.3f ms -> %.1f GB/s" % (gms, gather["achieved_GBs"]))
         gs.free()
 
@@ -326,11 +378,12 @@ def main():
                                "(a,b,c,d)=(0.57,0.19,0.19,0.05), seed %d, symmetrized, deduplicated, "
                                "degree-sorted ids, L0[v]=v" % (args.scale, args.edge_factor, args.seed),
                    "n": n, "m": m, "max_degree": info["max_degree"],
-                   "graph_replay": bool(args.graph_replay)},
+                   "graph_replay": bool(args.graph_replay), "tuning": tuned or "default"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "step_roofline": step_roofline,
         "gather": gather,
+        "frontier": frontier,
         "tiers_ms": {k: round(v["ms"], 4) for k, v in tiers.items()},
         "changes_per_step": changes[-args.steps:],
     }

@@ -120,6 +120,7 @@ struct HubArgs {
     int32_t part_keys;        // expected distinct labels per counting workgroup
     int32_t part_limit;       // distinct labels that trigger halving (0: table/2)
     int32_t nhubs;
+    int32_t nparts;           // (hub, bucket) parts; k_hub_part takes them in batches
     HubScratch hs;
 };
 
@@ -136,6 +137,7 @@ struct JobDesc {
     const int32_t* block[kMaxBlockClasses];
     HubArgs hub;
     int32_t ngroups, nwave;
+    int32_t ngroups_e;  // groups with edges (the first ngroups_e of the list)
     int32_t blocks[kLaunches];
 };
 

@@ -48,6 +48,12 @@ struct StepArgs {
     const uint8_t* act_r;
     uint8_t* act_w;
     const int32_t* fctl;
+    // frontier gather (separate gather with the frontier on; null otherwise):
+    // bit v of chg_w is set when v's label changes in this step; chg_r holds
+    // the previous step's bits, and when fctl[1] = 1 (that step changed few
+    // labels) the gather rewrites only the edges whose source bit is set
+    const uint32_t* chg_r;
+    uint32_t* chg_w;
 };
 
 __host__ __device__ __forceinline__ uint32_t order_i32(int32_t s) {

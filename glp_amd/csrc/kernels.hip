@@ -101,16 +101,39 @@ __device__ __forceinline__ void count_edges(const StepArgs& a, int32_t v, int32_
     const int lane = tid & 63;
     for (int32_t jw = b + (tid & ~63); jw < e; jw += STRIDE * K) {
         uint32_t key[K], fr[K];
+        // branch-free: an index past the end is clamped to the last edge (a
+        // valid address) and its key dropped afterwards, so the K neighbour
+        // loads and then the K label loads issue back to back; a guarded load
+        // would wait for every older load before issuing the next
 #pragma unroll
         for (int i = 0; i < K; ++i) {
             const int32_t j = jw + lane + i * STRIDE;
-            key[i] = 0u;
-            fr[i] = 0u;
-            if (j < e) key[i] = H::template load_neighbor<W, G>(a, v, j, &fr[i]);
+            key[i] = H::template load_neighbor<W, G>(a, v, min(j, e - 1), &fr[i]);
         }
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            if (jw + lane + i * STRIDE >= e) key[i] = 0u;
 #pragma unroll
         for (int i = 0; i < K; ++i) wave_insert<W>(Kt, Ct, mask, key[i], fr[i]);
     }
+}
+
+// Sets v's bit in the changed-vertex bitmap (frontier gather only).
+__device__ __forceinline__ void note_changed(const StepArgs& a, int32_t v) {
+    if (a.chg_w) atomicOr(&a.chg_w[v >> 5], 1u << (v & 31));
+}
+// The same for vertices v0 + i, i a set bit of `bal` (one wave's ballot);
+// one lane ORs the <= 3 words they span.
+__device__ __forceinline__ void note_changed_run(const StepArgs& a, int32_t v0,
+                                                 unsigned long long bal) {
+    if (!a.chg_w || bal == 0ull || lane_id() != 0) return;
+    const int32_t w0 = v0 >> 5;
+    const uint32_t sh = (uint32_t)v0 & 31u;
+    const unsigned long long lo = bal << sh;
+    const uint32_t hi = sh ? (uint32_t)(bal >> (64u - sh)) : 0u;
+    if ((uint32_t)lo) atomicOr(&a.chg_w[w0], (uint32_t)lo);
+    if ((uint32_t)(lo >> 32)) atomicOr(&a.chg_w[w0 + 1], (uint32_t)(lo >> 32));
+    if (hi) atomicOr(&a.chg_w[w0 + 2], hi);
 }
 
 __device__ __forceinline__ void count_changes(const StepArgs& a, bool changed) {
@@ -148,113 +171,213 @@ __device__ __forceinline__ void mark_vertex(const StepArgs& a, int32_t v, int32_
 // table of jobs folded into one launch (blk = index inside the owning job).
 
 // ---------------------------------------------------------------- packed ----
-template <class H, bool W, bool G>
-__device__ __forceinline__ void packed_body(const StepArgs& a, const int2* __restrict__ groups,
-                                            int32_t ngroups, uint32_t blk) {
-    const int lane = lane_id();
-    // wave-uniform group index (readfirstlane keeps the group loads scalar)
-    const int32_t gid = __builtin_amdgcn_readfirstlane((int32_t)((blk * 256u + threadIdx.x) >> 6));
-    if (gid >= ngroups) return;
-    const int2 gr = groups[gid];
-    const int32_t v0 = gr.x, vc = gr.y;
-    if (frontier_on<H>(a)) {
-        const bool act = lane < vc && a.act_r[v0 + lane] != 0;
-        if (__ballot(act) == 0ull) return;  // no vertex of the group needs work
-    }
-    const int32_t base = a.off[v0];
-    // vertex lane i: its edges are lanes [vb, ve) of the group
-    int32_t ve = 0;
-    if (lane < vc) ve = a.off[v0 + lane + 1] - base;
-    int32_t vb = shfl_i32(ve, lane > 0 ? lane - 1 : 0);
-    if (lane == 0) vb = 0;
-    const int32_t nE = shfl_i32(ve, vc - 1);
-    // owner of edge lane `lane`: smallest vertex slot i with ve_i > lane
-    int lo = 0, hi = vc - 1;
+// A wave takes kPackNG consecutive groups.  When all of them have edges (the
+// planner puts the edge-less groups last) their loads are issued branch-free,
+// every group's before any group is counted, so the dependent loads of the
+// groups (offsets, neighbour ids, labels) overlap; otherwise, and with the
+// frontier on, the groups go one by one through the guarded path.
+constexpr int kPackNG = 2;
+
+// One group's state between its loads and its count.  Vertex lane i owns
+// edge lanes [vb, ve) of the group; edge lane `lane` belongs to vertex slot
+// `own`, whose edges are lanes [sb, se).
+struct PackedGroup {
+    int32_t v0, vc, base, ve, vb, nE, own, sb, se, old;
+    uint32_t key, fr;
+};
+
+__device__ __forceinline__ void packed_segments(PackedGroup& s, int lane) {
+    s.vb = shfl_i32(s.ve, lane > 0 ? lane - 1 : 0);
+    if (lane == 0) s.vb = 0;
+    s.nE = shfl_i32(s.ve, s.vc - 1);
+    // owner: smallest vertex slot i with ve_i > lane
+    int lo = 0, hi = s.vc - 1;
 #pragma unroll
-    for (int s = 0; s < 6; ++s) {
+    for (int k = 0; k < 6; ++k) {
         const int mid = (lo + hi) >> 1;
-        const int32_t vm = shfl_i32(ve, mid);
+        const int32_t vm = shfl_i32(s.ve, mid);
         if (vm <= lane) lo = min(mid + 1, hi);
         else hi = mid;
     }
-    const int own = lo;
-    const int32_t sb = shfl_i32(vb, own), se = shfl_i32(ve, own);
-    const int32_t vid = v0 + own;
-    const bool live = lane < nE;
-    uint32_t key = 0u, fr = 0u;
-    if (live) key = H::template load_neighbor<W, G>(a, vid, base + lane, &fr);
+    s.own = lo;
+    s.sb = shfl_i32(s.vb, s.own);
+    s.se = shfl_i32(s.ve, s.own);
+}
+
+// Loads of a group with vc >= 1 vertices and nE >= 1 edges: indices past the
+// group are clamped to its last vertex / edge and the values dropped.
+template <class H, bool W, bool G>
+__device__ __forceinline__ void packed_load_fast(const StepArgs& a, int2 gr, PackedGroup& s) {
+    const int lane = lane_id();
+    s.v0 = gr.x;
+    s.vc = gr.y;
+    s.base = a.off[s.v0];
+    const int32_t li = min(lane, s.vc - 1);
+    const int32_t end = a.off[s.v0 + li + 1];
+    s.old = a.keep[s.v0 + li];
+    s.ve = lane < s.vc ? end - s.base : 0;
+    packed_segments(s, lane);
+    s.key = H::template load_neighbor<W, G>(a, s.v0 + s.own, s.base + min(lane, s.nE - 1), &s.fr);
+    if (lane >= s.nE) {
+        s.key = 0u;
+        s.fr = 0u;
+    }
+}
+
+// Guarded loads for any group; false when the frontier skips the group.
+template <class H, bool W, bool G>
+__device__ __forceinline__ bool packed_load_safe(const StepArgs& a, int2 gr, PackedGroup& s) {
+    const int lane = lane_id();
+    s.v0 = gr.x;
+    s.vc = gr.y;
+    if (frontier_on<H>(a)) {
+        const bool act = lane < s.vc && a.act_r[s.v0 + lane] != 0;
+        if (__ballot(act) == 0ull) return false;  // no vertex of the group needs work
+    }
+    s.base = a.off[s.v0];
+    s.ve = 0;
+    s.old = 0;
+    if (lane < s.vc) {
+        s.ve = a.off[s.v0 + lane + 1] - s.base;
+        s.old = a.keep[s.v0 + lane];
+    }
+    packed_segments(s, lane);
+    s.key = 0u;
+    s.fr = 0u;
+    if (lane < s.nE) s.key = H::template load_neighbor<W, G>(a, s.v0 + s.own, s.base + lane, &s.fr);
+    return true;
+}
+
+// Counts, picks and writes the group's labels (all lanes active).
+template <class H, bool W, bool G>
+__device__ __forceinline__ void packed_finish(const StepArgs& a, const PackedGroup& s) {
+    const int lane = lane_id();
+    const bool live = lane < s.nE;
     // frequency of my label inside my vertex's segment
-    const int32_t len = live ? se - sb : 0;
+    const int32_t len = live ? s.se - s.sb : 0;
     uint32_t cnt = 0u;
     for (int k = 0; __ballot(k < len) != 0ull; ++k) {
-        const int src = min(sb + k, 63);
-        const uint32_t ok = shfl_u32(key, src);
-        const uint32_t of = W ? shfl_u32(fr, src) : 1u;
-        if (k < len && ok == key) cnt += of;
+        const int src = min(s.sb + k, 63);
+        const uint32_t ok = shfl_u32(s.key, src);
+        const uint32_t of = W ? shfl_u32(s.fr, src) : 1u;
+        if (k < len && ok == s.key) cnt += of;
     }
-    uint64_t p = live ? H::label_score(a, vid, key, cnt) : 0ull;
+    const int32_t vid = s.v0 + s.own;
+    uint64_t p = live ? H::label_score(a, vid, s.key, cnt) : 0ull;
     // segmented max: after the sweep the last lane of a segment holds its max
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
         const uint64_t o = shfl_u64(p, lane >= d ? lane - d : lane);
-        if (lane - d >= sb) p = max_u64(p, o);
+        if (lane - d >= s.sb) p = max_u64(p, o);
     }
-    const bool has = lane < vc && ve > vb;
-    const uint64_t best = shfl_u64(p, has ? ve - 1 : lane);
+    const bool has = lane < s.vc && s.ve > s.vb;
+    const uint64_t best = shfl_u64(p, has ? s.ve - 1 : lane);
     bool changed = false;
-    if (lane < vc) {
-        const int32_t v = v0 + lane;
-        const int32_t old = a.keep[v];
-        const int32_t nl = has ? packed_label(best) : old;
+    if (lane < s.vc) {
+        const int32_t v = s.v0 + lane;
+        const int32_t nl = has ? packed_label(best) : s.old;
         H::update_vertex(a, v, nl, has ? best : 0ull);
-        changed = nl != old;
+        changed = nl != s.old;
     }
     if (marking<H>(a)) {
-        if (changed) a.act_w[v0 + lane] = 1;
-        const bool owner_changed = shfl_i32(changed ? 1 : 0, own) != 0;
-        if (live && owner_changed) a.act_w[a.nbr[base + lane]] = 1;
+        if (changed) a.act_w[s.v0 + lane] = 1;
+        const bool owner_changed = shfl_i32(changed ? 1 : 0, s.own) != 0;
+        if (live && owner_changed) a.act_w[a.nbr[s.base + lane]] = 1;
     }
+    note_changed_run(a, s.v0, __ballot(changed));
     count_changes(a, changed);
 }
 
+// ngroups: groups to run; groups [0, nge) have edges.
+template <class H, bool W, bool G>
+__device__ __forceinline__ void packed_body(const StepArgs& a, const int2* __restrict__ groups,
+                                            int32_t ngroups, int32_t nge, uint32_t blk) {
+    // wave-uniform first group (readfirstlane keeps the group loads scalar)
+    const int32_t g0 =
+        __builtin_amdgcn_readfirstlane((int32_t)((blk * 256u + threadIdx.x) >> 6)) * kPackNG;
+    if (g0 >= ngroups) return;
+    if (g0 + kPackNG <= nge && !frontier_on<H>(a)) {
+        PackedGroup s[kPackNG];
+#pragma unroll
+        for (int g = 0; g < kPackNG; ++g) packed_load_fast<H, W, G>(a, groups[g0 + g], s[g]);
+#pragma unroll
+        for (int g = 0; g < kPackNG; ++g) packed_finish<H, W, G>(a, s[g]);
+        return;
+    }
+    for (int g = 0; g < kPackNG && g0 + g < ngroups; ++g) {
+        PackedGroup s;
+        if (packed_load_safe<H, W, G>(a, groups[g0 + g], s)) packed_finish<H, W, G>(a, s);
+    }
+}
+
 // ------------------------------------------------------------------ wave ----
+// A wave takes kWaveNV vertices of the tier.  A vertex has at most 256 edges
+// (4 per lane), so one round of branch-free loads covers it: the loads of all
+// kWaveNV vertices are issued first, then each vertex is counted in the
+// wave's LDS table in turn.
+constexpr int kWaveNV = 2;
 template <class H, bool W, bool G>
 __device__ __forceinline__ void wave_body(const StepArgs& a, const int32_t* __restrict__ verts,
                                           int32_t nv, uint32_t blk) {
     __shared__ uint32_t s_key[4][kWaveCap];
     __shared__ uint32_t s_cnt[4][kWaveCap];
     const int lane = lane_id(), wv = threadIdx.x >> 6;
-    const int32_t idx = (int32_t)blk * 4 + wv;
-    if (idx >= nv) return;  // wave-uniform
-    const int32_t v = verts[idx];
-    if (!is_active<H>(a, v)) return;
-    const int32_t b = a.off[v], e = a.off[v + 1];
-    const uint32_t cap = min(max(next_pow2(2u * (uint32_t)(e - b)), 64u), (uint32_t)kWaveCap);
-    const uint32_t mask = cap - 1u;
+    const int32_t i0 = ((int32_t)blk * 4 + wv) * kWaveNV;
+    if (i0 >= nv) return;  // wave-uniform
+    int32_t v[kWaveNV], b[kWaveNV], e[kWaveNV], old[kWaveNV];
+    bool run[kWaveNV];
+    uint32_t key[kWaveNV][4], fr[kWaveNV][4];
+#pragma unroll
+    for (int k = 0; k < kWaveNV; ++k) v[k] = verts[min(i0 + k, nv - 1)];
+#pragma unroll
+    for (int k = 0; k < kWaveNV; ++k) {
+        run[k] = i0 + k < nv && is_active<H>(a, v[k]);
+        b[k] = a.off[v[k]];
+        e[k] = a.off[v[k] + 1];
+        old[k] = a.keep[v[k]];
+    }
+    // every tier vertex has >= 1 edge, so e-1 is a valid index; a vertex not
+    // run loads its first edge only (one line for the whole wave)
+#pragma unroll
+    for (int k = 0; k < kWaveNV; ++k)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int32_t j = b[k] + lane + 64 * i;
+            key[k][i] = H::template load_neighbor<W, G>(a, v[k], run[k] ? min(j, e[k] - 1) : b[k], &fr[k][i]);
+            if (j >= e[k]) key[k][i] = 0u;
+        }
     uint32_t* K = s_key[wv];
     uint32_t* C = s_cnt[wv];
-    for (uint32_t i = lane; i < cap; i += 64) {
-        K[i] = 0u;
-        C[i] = 0u;
+#pragma unroll
+    for (int k = 0; k < kWaveNV; ++k) {
+        if (!run[k]) continue;  // wave-uniform
+        const uint32_t cap = min(max(next_pow2(2u * (uint32_t)(e[k] - b[k])), 64u), (uint32_t)kWaveCap);
+        const uint32_t mask = cap - 1u;
+        for (uint32_t i = lane; i < cap; i += 64) {
+            K[i] = 0u;
+            C[i] = 0u;
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int i = 0; i < 4; ++i) wave_insert<W>(K, C, mask, key[k][i], fr[k][i]);
+        wave_lds_sync();
+        uint64_t best = 0ull;
+        for (uint32_t i = lane; i < cap; i += 64) {
+            const uint32_t kk = K[i];
+            if (kk) best = max_u64(best, H::label_score(a, v[k], kk, C[i]));
+        }
+        best = wave_max_u64(best);
+        wave_lds_sync();  // the next vertex clears what this scan read
+        bool changed = false;
+        if (lane == 0) {
+            const int32_t nl = packed_label(best);
+            H::update_vertex(a, v[k], nl, best);
+            changed = nl != old[k];
+            if (changed) note_changed(a, v[k]);
+        }
+        if (marking<H>(a) && shfl_i32(changed ? 1 : 0, 0) != 0) mark_vertex(a, v[k], b[k], e[k], lane, 64);
+        count_changes(a, changed);
     }
-    wave_lds_sync();
-    count_edges<H, W, G, 4, 64>(a, v, b, e, lane, K, C, mask);
-    wave_lds_sync();
-    uint64_t best = 0ull;
-    for (uint32_t i = lane; i < cap; i += 64) {
-        const uint32_t k = K[i];
-        if (k) best = max_u64(best, H::label_score(a, v, k, C[i]));
-    }
-    best = wave_max_u64(best);
-    bool changed = false;
-    if (lane == 0) {
-        const int32_t old = a.keep[v];
-        const int32_t nl = packed_label(best);
-        H::update_vertex(a, v, nl, best);
-        changed = nl != old;
-    }
-    if (marking<H>(a) && shfl_i32(changed ? 1 : 0, 0) != 0) mark_vertex(a, v, b, e, lane, 64);
-    count_changes(a, changed);
 }
 
 // ----------------------------------------------------------------- block ----
@@ -304,6 +427,7 @@ __device__ __forceinline__ void block_body(const StepArgs& a, const int32_t* __r
             const int32_t nl = packed_label(best);
             H::update_vertex(a, v, nl, best);
             changed = nl != old;
+            if (changed) note_changed(a, v);
             s_changed = changed ? 1u : 0u;
         }
         count_changes(a, changed);
@@ -468,92 +592,148 @@ __device__ __forceinline__ void part_pass(const unsigned long long* pr, uint32_t
                                           uint64_t hi, uint32_t* K, uint32_t* C, uint32_t shift,
                                           uint32_t limit, uint32_t* s_distinct, uint32_t* s_over) {
     const uint32_t mask = (1u << (32 - shift)) - 1u;
-    for (uint32_t i = threadIdx.x; i < n; i += kHubThreads) {
+    constexpr int kU = 4;  // pairs loaded per thread before any is inserted
+    for (uint32_t i0 = threadIdx.x; i0 < n; i0 += kHubThreads * kU) {
         if (lds_load(s_over)) return;
-        const unsigned long long x = pr[i];
-        const uint32_t key = (uint32_t)(x >> 32);
-        const uint32_t hk = fmix32(key);
-        if (hk < lo || hk >= hi) continue;
-        const uint32_t f = (uint32_t)x;
-        uint32_t s = (hk * 0x9e3779b1u) >> shift;
-        for (;;) {
-            const uint32_t k = lds_load(&K[s]);
-            if (k == key) {
-                atomicAdd(&C[s], f);
-                break;
-            }
-            if (k == 0u) {
-                const uint32_t prev = atomicCAS(&K[s], 0u, key);
-                if (prev == 0u) {
-                    if (atomicAdd(s_distinct, 1u) + 1u > limit) atomicExch(s_over, 1u);
+        unsigned long long xs[kU];
+        // clamped indices keep the loads branch-free (issued together)
+#pragma unroll
+        for (int u = 0; u < kU; ++u) xs[u] = pr[min(i0 + u * kHubThreads, n - 1u)];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (i0 + u * kHubThreads >= n) break;
+            const uint32_t key = (uint32_t)(xs[u] >> 32);
+            const uint32_t hk = fmix32(key);
+            if (hk < lo || hk >= hi) continue;
+            const uint32_t f = (uint32_t)xs[u];
+            uint32_t s = (hk * 0x9e3779b1u) >> shift;
+            for (;;) {
+                const uint32_t k = lds_load(&K[s]);
+                if (k == key) {
                     atomicAdd(&C[s], f);
                     break;
                 }
-                if (prev == key) {
-                    atomicAdd(&C[s], f);
-                    break;
+                if (k == 0u) {
+                    const uint32_t prev = atomicCAS(&K[s], 0u, key);
+                    if (prev == 0u) {
+                        if (atomicAdd(s_distinct, 1u) + 1u > limit) atomicExch(s_over, 1u);
+                        atomicAdd(&C[s], f);
+                        break;
+                    }
+                    if (prev == key) {
+                        atomicAdd(&C[s], f);
+                        break;
+                    }
                 }
+                s = (s + 1u) & mask;
             }
-            s = (s + 1u) & mask;
         }
     }
 }
 
-template <class H>
-__device__ __forceinline__ void hub_part_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
-    extern __shared__ uint32_t s_dyn[];
-    __shared__ unsigned long long s_red[kHubThreads / 64];
-    __shared__ uint32_t s_distinct, s_over;
-    const int2 hp = h.parts[blk];
-    const int32_t v = h.hubs[hp.x];
-    const int32_t d = a.off[v + 1] - a.off[v];
+// A (hub, bucket) part: the hub, its vertex, the bucket and its pair range in
+// the hub's bucket-sorted pairs.
+struct PartJob {
+    int32_t hub, v;
+    uint32_t q, P, start, stop;
+};
+
+__device__ __forceinline__ PartJob part_lookup(const StepArgs& a, const HubArgs& h, int32_t p) {
+    PartJob j;
+    const int2 hp = h.parts[p];
+    j.hub = hp.x;
+    j.v = h.hubs[hp.x];
+    const int32_t d = a.off[j.v + 1] - a.off[j.v];
     const int64_t nch = (d + h.chunk - 1) / h.chunk;
-    const uint32_t P = hub_buckets(d, h.part_keys);
-    const uint32_t q = (uint32_t)hp.y;
+    j.P = hub_buckets(d, h.part_keys);
+    j.q = (uint32_t)hp.y;
     const uint32_t* hist = h.hs.hist + h.hist_base[hp.x];
-    const uint32_t start = hist[(int64_t)q * nch];
-    const uint32_t stop = q + 1 < P ? hist[(int64_t)(q + 1) * nch] : h.hs.total[hp.x];
-    if (start == stop) return;  // block-uniform: empty bucket (or inactive hub)
-    const unsigned long long* pr = h.hs.sorted + h.base[hp.x] + start;
-    const uint32_t n = stop - start;
-    const uint32_t cap = next_pow2(4u * (uint32_t)h.part_keys);
+    j.start = hist[(int64_t)j.q * nch];
+    j.stop = j.q + 1 < j.P ? hist[(int64_t)(j.q + 1) * nch] : h.hs.total[hp.x];
+    return j;
+}
+
+// Counts one non-empty part in the LDS table at s_dyn, which is all zero on
+// entry and is left all zero (the scan clears what it reads).
+template <class H>
+__device__ __forceinline__ void hub_part_count(const StepArgs& a, const HubArgs& h, const PartJob& j,
+                                               unsigned long long* s_red, uint32_t* s_distinct,
+                                               uint32_t* s_over) {
+    extern __shared__ uint32_t s_dyn[];
+    const unsigned long long* pr = h.hs.sorted + h.base[j.hub] + j.start;
+    const uint32_t n = j.stop - j.start;
+    const uint32_t cap_max = next_pow2(4u * (uint32_t)h.part_keys);
+    // a bucket of n <= cap_max/2 pairs gets a table of >= 2n slots: it holds
+    // at most n distinct labels, so it cannot overflow, and converged buckets
+    // (a few pairs) scan a few slots instead of the whole table
+    uint32_t cap = cap_max;
+    if (h.part_limit == 0 && n <= cap_max / 2u) cap = max(next_pow2(2u * n), 64u);
     const uint32_t shift = 32u - (31u - __builtin_clz(cap));
     uint32_t* K = s_dyn;
     uint32_t* C = s_dyn + cap;
-    const uint64_t end = bucket_lo(q + 1, P);
-    uint64_t lo = bucket_lo(q, P), hi = end;
+    const uint64_t end = bucket_lo(j.q + 1, j.P);
+    uint64_t lo = bucket_lo(j.q, j.P), hi = end;
     const uint32_t limit = h.part_limit > 0 ? (uint32_t)h.part_limit : cap / 2u;
     uint64_t best = 0ull;
     while (lo < end) {
-        for (uint32_t i = threadIdx.x; i < cap; i += kHubThreads) {
-            K[i] = 0u;
-            C[i] = 0u;
-        }
         if (threadIdx.x == 0) {
-            s_distinct = 0u;
-            s_over = 0u;
+            *s_distinct = 0u;
+            *s_over = 0u;
         }
         __syncthreads();
-        part_pass(pr, n, lo, hi, K, C, shift, limit, &s_distinct, &s_over);
+        part_pass(pr, n, lo, hi, K, C, shift, limit, s_distinct, s_over);
         __syncthreads();
-        const bool over = s_over != 0u;
-        __syncthreads();
-        if (over) {
-            // retry the lower half of the range (a range no wider than `limit`
-            // cannot overflow, so the width stays >= 1)
+        if (*s_over != 0u) {
+            // the pass stopped part-way: clear the table and retry the lower
+            // half of the range (a range no wider than `limit` cannot
+            // overflow, so the width stays >= 1)
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < 2u * cap; i += kHubThreads) s_dyn[i] = 0u;
             hi = lo + max((hi - lo) / 2, (uint64_t)1);
             continue;
         }
         for (uint32_t i = threadIdx.x; i < cap; i += kHubThreads) {
             const uint32_t k = K[i];
-            if (k) best = max_u64(best, H::label_score(a, v, k, C[i]));
+            if (k) {
+                best = max_u64(best, H::label_score(a, j.v, k, C[i]));
+                K[i] = 0u;
+                C[i] = 0u;
+            }
         }
-        __syncthreads();
         lo = hi;
         hi = end;
     }
-    best = block_max_u64<kHubThreads>(best, s_red);
-    if (threadIdx.x == 0 && best) atomicMax(&h.hs.best[hp.x], (unsigned long long)best);
+    best = block_max_u64<kHubThreads>(best, s_red);  // its barrier also ends the scan
+    if (threadIdx.x == 0 && best) atomicMax(&h.hs.best[j.hub], (unsigned long long)best);
+}
+
+// One workgroup per batch of kPartBatch parts: each thread looks up one part,
+// the non-empty ones are listed in LDS and counted one after another.  Once
+// labels converge nearly every bucket is empty, and a launch of one
+// workgroup per part would spend its time starting and ending them.
+constexpr int kPartBatch = kHubThreads;
+
+template <class H>
+__device__ __forceinline__ void hub_part_batch_body(const StepArgs& a, const HubArgs& h,
+                                                    uint32_t blk) {
+    extern __shared__ uint32_t s_dyn[];
+    __shared__ PartJob s_jobs[kPartBatch];
+    __shared__ unsigned long long s_red[kHubThreads / 64];
+    __shared__ uint32_t s_n, s_distinct, s_over;
+    if (threadIdx.x == 0) s_n = 0u;
+    __syncthreads();
+    const int32_t p = (int32_t)(blk * kPartBatch + threadIdx.x);
+    if (p < h.nparts) {
+        const PartJob j = part_lookup(a, h, p);
+        if (j.start != j.stop) s_jobs[atomicAdd(&s_n, 1u)] = j;
+    }
+    __syncthreads();
+    const uint32_t n = s_n;
+    if (n == 0u) return;  // block-uniform
+    const uint32_t words = 2u * next_pow2(4u * (uint32_t)h.part_keys);
+    for (uint32_t i = threadIdx.x; i < words; i += kHubThreads) s_dyn[i] = 0u;
+    // (the first count's barrier orders this clear before its inserts)
+    for (uint32_t i = 0; i < n; ++i) hub_part_count<H>(a, h, s_jobs[i], s_red, &s_distinct, &s_over);
 }
 
 template <class H>
@@ -569,6 +749,7 @@ __device__ __forceinline__ void hub_final_body(const StepArgs& a, const HubArgs&
             const int32_t nl = packed_label(best);
             H::update_vertex(a, v, nl, best);
             changed = nl != old;
+            if (changed) note_changed(a, v);
         }
         if (marking<H>(a)) {
             h.hs.changed[i] = changed ? 1u : 0u;
@@ -595,8 +776,8 @@ __device__ __forceinline__ void hub_mark_body(const StepArgs& a, const HubArgs& 
 // Single job: workgroup index = blockIdx.x.
 template <class H, bool W, bool G>
 __global__ __launch_bounds__(256) void k_packed(StepArgs a, const int2* __restrict__ groups,
-                                                int32_t ngroups) {
-    packed_body<H, W, G>(a, groups, ngroups, blockIdx.x);
+                                                int32_t ngroups, int32_t nge) {
+    packed_body<H, W, G>(a, groups, ngroups, nge, blockIdx.x);
 }
 template <class H, bool W, bool G>
 __global__ __launch_bounds__(256) void k_wave(StepArgs a, const int32_t* __restrict__ verts,
@@ -620,7 +801,7 @@ __global__ __launch_bounds__(kHubThreads) void k_hub_scatter(StepArgs a, HubArgs
 }
 template <class H>
 __global__ __launch_bounds__(kHubThreads) void k_hub_part(StepArgs a, HubArgs h) {
-    hub_part_body<H>(a, h, blockIdx.x);
+    hub_part_batch_body<H>(a, h, blockIdx.x);
 }
 template <class H>
 __global__ __launch_bounds__(256) void k_hub_final(StepArgs a, HubArgs h) {
@@ -652,7 +833,7 @@ __global__ __launch_bounds__(NT) void k_multi(const JobDesc* __restrict__ jobs,
     const uint32_t blk = blockIdx.x - (uint32_t)cum[j];
     const JobDesc& jd = jobs[j];
     if constexpr (L == kLaunchPacked) {
-        packed_body<H, W, G>(jd.a, jd.groups, jd.ngroups, blk);
+        packed_body<H, W, G>(jd.a, jd.groups, jd.ngroups, jd.ngroups_e, blk);
     } else if constexpr (L == kLaunchWave) {
         wave_body<H, W, G>(jd.a, jd.wave, jd.nwave, blk);
     } else if constexpr (L >= kLaunchBlock0 && L < kLaunchBlock0 + kMaxBlockClasses) {
@@ -664,7 +845,7 @@ __global__ __launch_bounds__(NT) void k_multi(const JobDesc* __restrict__ jobs,
     } else if constexpr (L == kLaunchHubScatter) {
         hub_scatter_body(jd.a, jd.hub, blk);
     } else if constexpr (L == kLaunchHubPart) {
-        hub_part_body<H>(jd.a, jd.hub, blk);
+        hub_part_batch_body<H>(jd.a, jd.hub, blk);
     } else {
         hub_final_body<H>(jd.a, jd.hub, blk);
     }
@@ -735,13 +916,13 @@ int launch_multi(bool weighted, const JobDesc* jobs, const int32_t* cum, int32_t
 // Workgroups each launch of a folded step needs for one job.
 void job_blocks(const Plan& p, int32_t* blocks) {
     for (int L = 0; L < kLaunches; ++L) blocks[L] = 0;
-    blocks[kLaunchPacked] = (int32_t)((p.groups.size() + 3) / 4);
-    blocks[kLaunchWave] = (int32_t)((p.wave.size() + 3) / 4);
+    blocks[kLaunchPacked] = (int32_t)((p.groups.size() + 4 * kPackNG - 1) / (4 * kPackNG));
+    blocks[kLaunchWave] = (int32_t)((p.wave.size() + 4 * kWaveNV - 1) / (4 * kWaveNV));
     for (int c = 0; c < p.nclasses; ++c) blocks[kLaunchBlock0 + c] = (int32_t)p.block[c].size();
     blocks[kLaunchHubAgg] = (int32_t)p.hub_chunks.size();
     blocks[kLaunchHubScan] = (int32_t)p.hubs.size();
     blocks[kLaunchHubScatter] = (int32_t)p.hub_chunks.size();
-    blocks[kLaunchHubPart] = (int32_t)p.hub_parts.size();
+    blocks[kLaunchHubPart] = (int32_t)((p.hub_parts.size() + kPartBatch - 1) / kPartBatch);
     blocks[kLaunchHubFinal] = (int32_t)((p.hubs.size() + 255) / 256);
 }
 
@@ -755,9 +936,9 @@ __global__ __launch_bounds__(256) void k_pick(StepArgs a, int32_t* __restrict__ 
 // adj[j] = L[nbr[j]] (header/kernel.cuh:225-256 gather_labels), four
 // neighbours per lane with 16-byte streaming loads and stores.
 typedef int32_t v4i32 __attribute__((ext_vector_type(4)));
-__global__ __launch_bounds__(256) void k_gather(const int32_t* __restrict__ nbr,
-                                                const int32_t* __restrict__ L,
-                                                int32_t* __restrict__ adj, int64_t m) {
+__device__ __forceinline__ void gather_all(const int32_t* __restrict__ nbr,
+                                           const int32_t* __restrict__ L,
+                                           int32_t* __restrict__ adj, int64_t m) {
     const int64_t nvec = m >> 2;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const v4i32* nb4 = reinterpret_cast<const v4i32*>(nbr);
@@ -773,6 +954,44 @@ __global__ __launch_bounds__(256) void k_gather(const int32_t* __restrict__ nbr,
     }
     const int64_t t = (nvec << 2) + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x == 0 && t < m) adj[t] = L[nbr[t]];
+}
+
+__global__ __launch_bounds__(256) void k_gather(const int32_t* __restrict__ nbr,
+                                                const int32_t* __restrict__ L,
+                                                int32_t* __restrict__ adj, int64_t m) {
+    gather_all(nbr, L, adj, m);
+}
+
+// Frontier gather (header/kernel.cuh:21-35's if_update gather, with a bitmap
+// of the sources that changed in the previous step).  adj already holds the
+// labels of two steps back, so rewriting the edges whose source changed makes
+// it equal L[nbr]; after a step that changed many labels (fctl[1] = 0, also
+// the first step) every edge is rewritten.
+__global__ __launch_bounds__(256) void k_gather_frontier(const int32_t* __restrict__ nbr,
+                                                         const int32_t* __restrict__ L,
+                                                         int32_t* __restrict__ adj, int64_t m,
+                                                         const uint32_t* __restrict__ chg,
+                                                         const int32_t* __restrict__ fctl) {
+    if (fctl[1] == 0) {
+        gather_all(nbr, L, adj, m);
+        return;
+    }
+    const int64_t nvec = m >> 2;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const v4i32* nb4 = reinterpret_cast<const v4i32*>(nbr);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += stride) {
+        const v4i32 u = __builtin_nontemporal_load(&nb4[i]);
+        const int64_t j = i << 2;
+        if ((chg[u.x >> 5] >> (u.x & 31)) & 1u) adj[j] = L[u.x];
+        if ((chg[u.y >> 5] >> (u.y & 31)) & 1u) adj[j + 1] = L[u.y];
+        if ((chg[u.z >> 5] >> (u.z & 31)) & 1u) adj[j + 2] = L[u.z];
+        if ((chg[u.w >> 5] >> (u.w & 31)) & 1u) adj[j + 3] = L[u.w];
+    }
+    const int64_t t = (nvec << 2) + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && t < m) {
+        const int32_t u = nbr[t];
+        if ((chg[u >> 5] >> (u & 31)) & 1u) adj[t] = L[u];
+    }
 }
 
 __global__ void k_fill_i32(int32_t* p, int32_t value, int64_t count) {
@@ -831,15 +1050,15 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
     mark(GLP_TIER_PACKED, 0);
     const int32_t ng = (int32_t)(H::kSkipIsolated ? p.groups_with_edges : (int64_t)p.groups.size());
     if (ng > 0) {
-        hipLaunchKernelGGL((k_packed<H, W, G>), dim3(grid_for(ng, 4)), dim3(256), 0, s, a,
-                           g.d_groups, ng);
+        hipLaunchKernelGGL((k_packed<H, W, G>), dim3(grid_for(ng, 4 * kPackNG)), dim3(256), 0, s, a,
+                           g.d_groups, ng, (int32_t)p.groups_with_edges);
     }
     mark(GLP_TIER_PACKED, 1);
     mark(GLP_TIER_WAVE, 0);
     if (!p.wave.empty()) {
         const int32_t nw = (int32_t)p.wave.size();
-        hipLaunchKernelGGL((k_wave<H, W, G>), dim3(grid_for(nw, 4)), dim3(256), 0, s, a, g.d_wave,
-                           nw);
+        hipLaunchKernelGGL((k_wave<H, W, G>), dim3(grid_for(nw, 4 * kWaveNV)), dim3(256), 0, s, a,
+                           g.d_wave, nw);
     }
     mark(GLP_TIER_WAVE, 1);
     mark(GLP_TIER_BLOCK, 0);
@@ -873,6 +1092,7 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
         h.part_keys = p.tune.part_keys;
         h.part_limit = p.tune.part_limit;
         h.nhubs = (int32_t)p.hubs.size();
+        h.nparts = (int32_t)p.hub_parts.size();
         h.hs = hs;
         const unsigned nc = (unsigned)p.hub_chunks.size();
         const unsigned np = (unsigned)p.hub_parts.size();
@@ -883,7 +1103,8 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
         mark(GLP_TIER_AUX, 0);
         hipLaunchKernelGGL(k_hub_scan, dim3(h.nhubs), dim3(kHubThreads), 0, s, a, h);
         hipLaunchKernelGGL(k_hub_scatter, dim3(nc), dim3(kHubThreads), 0, s, a, h);
-        hipLaunchKernelGGL((k_hub_part<H>), dim3(np), dim3(kHubThreads), lds_part, s, a, h);
+        hipLaunchKernelGGL((k_hub_part<H>), dim3((np + kPartBatch - 1) / kPartBatch), dim3(kHubThreads),
+                           lds_part, s, a, h);
         hipLaunchKernelGGL((k_hub_final<H>), dim3(grid_for(h.nhubs, 256)), dim3(256), 0, s, a, h);
         if (H::kFrontier && a.act_w)
             hipLaunchKernelGGL((k_hub_mark<H>), dim3(nc), dim3(256), 0, s, a, h);
@@ -983,7 +1204,11 @@ int launch_pick(Method method, const StepArgs& a, int32_t* shown, hipStream_t s)
 int launch_gather(const StepArgs& a, int64_t m, int32_t* adj, hipStream_t s) {
     if (m == 0) return GLP_OK;
     const unsigned grid = (unsigned)std::min<int64_t>(grid_for(m / 4 + 1, 256), 256 * 64);
-    hipLaunchKernelGGL(k_gather, dim3(grid), dim3(256), 0, s, a.nbr, a.L, adj, m);
+    if (a.chg_r && a.fctl)
+        hipLaunchKernelGGL(k_gather_frontier, dim3(grid), dim3(256), 0, s, a.nbr, a.L, adj, m, a.chg_r,
+                           a.fctl);
+    else
+        hipLaunchKernelGGL(k_gather, dim3(grid), dim3(256), 0, s, a.nbr, a.L, adj, m);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "gather launch: %s", hipGetErrorString(e));
 }

@@ -184,6 +184,7 @@ int glp_server::tick() {
         weighted |= j->p.weighted != 0;
         d.groups = dg.d_groups;
         d.ngroups = (int32_t)p.groups.size();
+        d.ngroups_e = (int32_t)p.groups_with_edges;
         d.wave = dg.d_wave;
         d.nwave = (int32_t)p.wave.size();
         for (int c = 0; c < kMaxBlockClasses; ++c) d.block[c] = dg.d_block[c];
@@ -198,6 +199,7 @@ int glp_server::tick() {
         h.part_keys = tune.part_keys;
         h.part_limit = tune.part_limit;
         h.nhubs = (int32_t)p.hubs.size();
+        h.nparts = (int32_t)p.hub_parts.size();
         h.hs = j->hs;
         job_blocks(p, d.blocks);
     }

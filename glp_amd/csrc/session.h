@@ -25,6 +25,8 @@ struct glp_session {
     uint8_t* act[2] = {nullptr, nullptr};  // frontier flags, double-buffered like L
     int32_t* fctl = nullptr;             // frontier control {all_active, mark, threshold}
     int64_t act_bytes = 0;
+    uint32_t* chg[2] = {nullptr, nullptr};  // frontier gather: changed-vertex bitmaps
+    int64_t chg_bytes = 0;
     glp::HubScratch hs;
     unsigned long long* d_acc = nullptr;   // change counter of the running step
     unsigned long long* d_ring = nullptr;  // change count of each finished step

@@ -194,6 +194,8 @@ glp_session::~glp_session() {
     (void)hipFree(hs.changed);
     (void)hipFree(act[0]);
     (void)hipFree(act[1]);
+    (void)hipFree(chg[0]);
+    (void)hipFree(chg[1]);
     (void)hipFree(fctl);
     (void)hipFree(hs.best);
     (void)hipFree(d_acc);
@@ -217,6 +219,8 @@ size_t session_bytes(const HostCSR& g, const Plan& p, const glp_params& prm) {
     if (prm.method == GLP_SLPA) b += n * 4 + (size_t)(prm.slpa_T + 1) * n * 4;
     if (prm.method == GLP_USER) b += n * 4;
     if (prm.gather) b += (size_t)g.m * 4;
+    if (prm.frontier) b += 2 * (n + 4);
+    if (prm.frontier && prm.gather) b += 2 * ((n + 31) / 32) * 4;
     b += (size_t)p.pair_slots * 16 + (size_t)p.hist_slots * 4 + p.hub_chunks.size() * 4 +
          p.hubs.size() * 12;
     b += 8 * kChangeSlots + 8 * kChangeRing + 4;
@@ -270,6 +274,11 @@ int session_create(glp_graph* g, const glp_params* prm, glp_session** out) {
         GLP_HIP(hipMalloc(&s->act[0], (size_t)s->act_bytes));
         GLP_HIP(hipMalloc(&s->act[1], (size_t)s->act_bytes));
         GLP_HIP(hipMalloc(&s->fctl, 4 * sizeof(int32_t)));
+        if (prm->gather) {
+            s->chg_bytes = (((int64_t)nn + 31) / 32) * 4;
+            GLP_HIP(hipMalloc(&s->chg[0], (size_t)s->chg_bytes));
+            GLP_HIP(hipMalloc(&s->chg[1], (size_t)s->chg_bytes));
+        }
     }
     GLP_HIP(hipMalloc(&s->d_acc, 8 * kChangeSlots));
     GLP_HIP(hipMalloc(&s->d_ring, 8 * kChangeRing));
@@ -315,6 +324,8 @@ int session_reset(glp_session* s) {
         if ((rc = launch_zero_bytes(s->act[0], s->act_bytes, s->stream))) return rc;
         if ((rc = launch_zero_bytes(s->act[1], s->act_bytes, s->stream))) return rc;
     }
+    for (uint32_t* c : s->chg)
+        if (c && (rc = launch_zero_bytes(c, s->chg_bytes, s->stream))) return rc;
     GLP_HIP(hipMemsetAsync(s->d_acc, 0, 8 * kChangeSlots, s->stream));
     GLP_HIP(hipMemsetAsync(s->d_ring, 0, 8 * kChangeRing, s->stream));
     GLP_HIP(hipMemsetAsync(s->d_ctr, 0, 4, s->stream));
@@ -368,6 +379,10 @@ static StepArgs step_args(glp_session* s) {
         a.act_w = s->act[c ^ 1];
         a.fctl = s->fctl;
     }
+    if (s->chg[0]) {
+        a.chg_r = s->chg[c];
+        a.chg_w = s->chg[c ^ 1];
+    }
     return a;
 }
 
@@ -382,6 +397,8 @@ int enqueue_step(glp_session* s, hipEvent_t* ev) {
         return rc;
     // frontier flags for the next step start clear (they held the last step's)
     if (s->fctl && (rc = launch_zero_bytes(s->act[s->cur ^ 1], s->act_bytes, s->stream))) return rc;
+    // this step's changed-vertex bits start clear (they held two steps back)
+    if (s->chg[0] && (rc = launch_zero_bytes(s->chg[s->cur ^ 1], s->chg_bytes, s->stream))) return rc;
     if (ev) (void)hipEventRecord(ev[2 * GLP_TIER_PICK], s->stream);
     if (s->method == Method::Slpa || s->method == Method::User) {
         if ((rc = launch_pick(s->method, a, s->shown, s->stream))) return rc;

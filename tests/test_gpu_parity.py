@@ -186,6 +186,34 @@ def test_frontier_equals_dense(scale, tiny):
     assert ch[-1] < (len(off) - 1) // 32   # the frontier was in use at the end
 
 
+@pytest.mark.parametrize("scale,tiny,weighted", [(14, False, False), (13, True, False),
+                                                 (13, False, True)])
+def test_frontier_gather_equals_dense(scale, tiny, weighted):
+    """Separate gather with the frontier: after a step that changed at most
+    n/32 labels the gather rewrites only the edges whose source changed
+    (bitmap), otherwise all of them; every step must equal the oracle, also
+    under graph replay and across a checkpoint (which forces a full gather)."""
+    kw = dict(pack_max_deg=8, wave_max_deg=16, block_max_deg=32, hub_chunk=256,
+              part_keys=256) if tiny else {}
+    with G.tuning(**kw):
+        g = G.Graph.rmat(scale, 16, seed=70 + scale, wmax=8 if weighted else 0)
+        off, nbr, w = g.csr()
+        want = O.classic(off, nbr, 16, w=w if weighted else None)
+        _, dump, _ = G.run(g, 16, dump=True, frontier=True, gather=True, weighted=weighted)
+        assert_steps_equal(dump, want)
+        with G.Session(g, frontier=True, gather=True, weighted=weighted) as s:
+            s.step(3)
+            s.step(8, graph=True)
+            assert np.array_equal(s.labels(), want[10])
+            state = s.save()
+            ch = s.changes()
+        with G.Session(g, frontier=True, gather=True, weighted=weighted) as s2:
+            s2.load(state)
+            s2.step(5)
+            assert np.array_equal(s2.labels(), want[15])
+    assert ch[-1] < (len(off) - 1) // 32   # the partial gather was in use
+
+
 def test_frontier_rejected_for_layered():
     g = G.Graph.rmat(10, 8, seed=1)
     with pytest.raises(G.GLPError):
