@@ -24,6 +24,7 @@ struct Candidates {
 
 const Candidates kSpace[] = {
     {"pack_max_deg", {16, 24, 32, 48, 64}},
+    {"half_max_deg", {0, 48, 64}},
     {"wave_max_deg", {64, 128, 192, 256}},
     {"block_max_deg", {512, 1024, 2048, 4096, 8192}},
     {"block_threads", {256, 512, 1024}},
@@ -128,7 +129,7 @@ extern "C" int glp_autotune(glp_graph* g, const glp_params* p, int32_t steps, co
     if (out_path) {
         std::ofstream out(out_path);
         if (!out) return set_error(GLP_E_IO, "cannot write '%s'", out_path);
-        for (const char* k : {"pack_max_deg", "wave_max_deg", "block_max_deg", "hub_chunk", "part_keys",
+        for (const char* k : {"pack_max_deg", "half_max_deg", "wave_max_deg", "block_max_deg", "hub_chunk", "part_keys",
                               "part_limit", "block_threads"}) {
             int32_t v = 0;
             tuning_get(cur, k, &v);

@@ -21,6 +21,7 @@ namespace glp {
 // Degree thresholds and launch shapes; the autotuner writes these.
 struct Tuning {
     int32_t pack_max_deg = 32;     // deg <= this: lane-packed groups of <= 64 edges
+    int32_t half_max_deg = 64;     // deg <= this (<= 64): two vertices per wavefront (0: off)
     int32_t wave_max_deg = 256;    // deg <= this: one wavefront per vertex (LDS table)
     int32_t block_max_deg = 4096;  // deg <= this: one workgroup per vertex (LDS table)
     int32_t hub_chunk = 2048;      // edges per aggregation workgroup for hubs (deg > block_max_deg)
@@ -32,8 +33,9 @@ int tuning_set(Tuning& t, const char* key, int32_t value);  // GLP_E_ARG on unkn
 int tuning_get(const Tuning& t, const char* key, int32_t* value);
 Tuning& default_tuning();  // process-wide defaults (GLP_TUNING env file overrides)
 
-constexpr int kChangeSlots = 256;  // change counters, spread to avoid one hot address
+constexpr int kChangeSlots = 1024;  // change counters, spread to avoid one hot address
 constexpr int kWaveCap = 512;      // LDS slots per wavefront in the wave tier
+constexpr int kHalfMaxDeg = 64;    // largest degree of the half-wave part of the wave tier
 constexpr int kMaxBlockClasses = 5;
 
 // Host-built tier plan.  Vertex lists hold ids; groups hold (first id, count).
@@ -41,7 +43,8 @@ struct Plan {
     Tuning tune;
     std::vector<int2> groups;                 // packed tier (edge-less groups last)
     int64_t groups_with_edges = 0;            // groups before the edge-less ones
-    std::vector<int32_t> wave;                // wave tier
+    std::vector<int32_t> half;                // wave tier, deg <= half_max_deg (half-wave each)
+    std::vector<int32_t> wave;                // wave tier, larger degrees
     int32_t nclasses = 0;
     int32_t class_cap[kMaxBlockClasses] = {};  // LDS slots of each block class
     std::vector<int32_t> block[kMaxBlockClasses];
@@ -66,6 +69,7 @@ struct DeviceGraph {
     uint32_t* w = nullptr;
     Plan plan;
     int2* d_groups = nullptr;
+    int32_t* d_half = nullptr;
     int32_t* d_wave = nullptr;
     int32_t* d_block[kMaxBlockClasses] = {};
     int32_t* d_hubs = nullptr;
@@ -127,16 +131,17 @@ struct HubArgs {
 // One job's share of a folded (multi-job) step: its step arguments and
 // tier work lists.  blocks[] holds the workgroups it needs in each launch,
 // in the order LAUNCH_* below.
-enum { kLaunchPacked = 0, kLaunchWave = 1, kLaunchBlock0 = 2,
-       kLaunchHubAgg = 2 + kMaxBlockClasses, kLaunchHubScan, kLaunchHubScatter, kLaunchHubPart,
+enum { kLaunchPacked = 0, kLaunchHalf = 1, kLaunchWave = 2, kLaunchBlock0 = 3,
+       kLaunchHubAgg = 3 + kMaxBlockClasses, kLaunchHubScan, kLaunchHubScatter, kLaunchHubPart,
        kLaunchHubFinal, kLaunches };
 struct JobDesc {
     StepArgs a;
     const int2* groups;
+    const int32_t* half;
     const int32_t* wave;
     const int32_t* block[kMaxBlockClasses];
     HubArgs hub;
-    int32_t ngroups, nwave;
+    int32_t ngroups, nhalf, nwave;
     int32_t ngroups_e;  // groups with edges (the first ngroups_e of the list)
     int32_t blocks[kLaunches];
 };

@@ -136,10 +136,15 @@ __device__ __forceinline__ void note_changed_run(const StepArgs& a, int32_t v0,
     if (hi) atomicOr(&a.chg_w[w0 + 2], hi);
 }
 
+// Adds n changed labels (lane 0's n) to one of kChangeSlots counters chosen
+// by workgroup and wave, so no single address takes every wave's atomic.
+__device__ __forceinline__ void add_changes(const StepArgs& a, uint32_t n) {
+    if (lane_id() == 0 && n)
+        atomicAdd(&a.changes[(blockIdx.x * 16u + (threadIdx.x >> 6)) & (kChangeSlots - 1)],
+                  (unsigned long long)n);
+}
 __device__ __forceinline__ void count_changes(const StepArgs& a, bool changed) {
-    unsigned long long bal = __ballot(changed);
-    if (lane_id() == 0 && bal)
-        atomicAdd(&a.changes[blockIdx.x & (kChangeSlots - 1)], (unsigned long long)__popcll(bal));
+    add_changes(a, (uint32_t)__popcll(__ballot(changed)));
 }
 
 // Frontier helpers (classic LP; header/kernel.cuh:21-35's if_update idea):
@@ -248,9 +253,10 @@ __device__ __forceinline__ bool packed_load_safe(const StepArgs& a, int2 gr, Pac
     return true;
 }
 
-// Counts, picks and writes the group's labels (all lanes active).
+// Counts, picks and writes the group's labels (all lanes active); returns
+// how many of them changed.
 template <class H, bool W, bool G>
-__device__ __forceinline__ void packed_finish(const StepArgs& a, const PackedGroup& s) {
+__device__ __forceinline__ uint32_t packed_finish(const StepArgs& a, const PackedGroup& s) {
     const int lane = lane_id();
     const bool live = lane < s.nE;
     // frequency of my label inside my vertex's segment
@@ -284,8 +290,9 @@ __device__ __forceinline__ void packed_finish(const StepArgs& a, const PackedGro
         const bool owner_changed = shfl_i32(changed ? 1 : 0, s.own) != 0;
         if (live && owner_changed) a.act_w[a.nbr[s.base + lane]] = 1;
     }
-    note_changed_run(a, s.v0, __ballot(changed));
-    count_changes(a, changed);
+    const unsigned long long bal = __ballot(changed);
+    note_changed_run(a, s.v0, bal);
+    return (uint32_t)__popcll(bal);
 }
 
 // ngroups: groups to run; groups [0, nge) have edges.
@@ -296,18 +303,144 @@ __device__ __forceinline__ void packed_body(const StepArgs& a, const int2* __res
     const int32_t g0 =
         __builtin_amdgcn_readfirstlane((int32_t)((blk * 256u + threadIdx.x) >> 6)) * kPackNG;
     if (g0 >= ngroups) return;
+    uint32_t nch = 0u;
     if (g0 + kPackNG <= nge && !frontier_on<H>(a)) {
         PackedGroup s[kPackNG];
 #pragma unroll
         for (int g = 0; g < kPackNG; ++g) packed_load_fast<H, W, G>(a, groups[g0 + g], s[g]);
 #pragma unroll
-        for (int g = 0; g < kPackNG; ++g) packed_finish<H, W, G>(a, s[g]);
-        return;
+        for (int g = 0; g < kPackNG; ++g) nch += packed_finish<H, W, G>(a, s[g]);
+    } else {
+        for (int g = 0; g < kPackNG && g0 + g < ngroups; ++g) {
+            PackedGroup s;
+            if (packed_load_safe<H, W, G>(a, groups[g0 + g], s)) nch += packed_finish<H, W, G>(a, s);
+        }
     }
-    for (int g = 0; g < kPackNG && g0 + g < ngroups; ++g) {
-        PackedGroup s;
-        if (packed_load_safe<H, W, G>(a, groups[g0 + g], s)) packed_finish<H, W, G>(a, s);
+    add_changes(a, nch);
+}
+
+// ------------------------------------------------------------------ half ----
+// Wave-tier vertices of degree <= half_max_deg (<= kHalfMaxDeg): a wave
+// counts two at a time, lanes 0-31 one vertex and lanes 32-63 the other, two
+// edges per lane, each half in its own LDS table of kHalfCap slots.  A wave
+// takes kHalfPairs such pairs, their loads issued before any is counted.
+constexpr int kHalfPairs = 2;
+constexpr uint32_t kHalfCap = 2 * kHalfMaxDeg;
+
+// Reductions inside each 32-lane half (xor distances below 32).
+__device__ __forceinline__ uint64_t half_max_u64(uint64_t x) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 16; d >= 1; d >>= 1) x = max_u64(x, shfl_u64(x, lane ^ d));
+    return x;
+}
+__device__ __forceinline__ uint32_t half_sum_u32(uint32_t x) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 16; d >= 1; d >>= 1) x += shfl_u32(x, lane ^ d);
+    return x;
+}
+
+// wave_insert for two independent halves (Kt/Ct point at the lane's half's
+// table): each half merges the lanes holding its first pending lane's key.
+template <bool W>
+__device__ __forceinline__ void half_insert(uint32_t* Kt, uint32_t* Ct, uint32_t key, uint32_t f) {
+    const int lane = lane_id();
+    const bool upper = lane >= 32;
+    uint32_t ik = key, iff = f;  // what this lane finally inserts
+#pragma unroll
+    for (int r = 0; r < kPeels; ++r) {
+        const unsigned long long rest = __ballot(key != 0u);
+        if (rest == 0ull) break;
+        const uint32_t rlo = (uint32_t)rest, rhi = (uint32_t)(rest >> 32);
+        const int flo = rlo ? (int)__builtin_ctz(rlo) : 0;
+        const int fhi = rhi ? 32 + (int)__builtin_ctz(rhi) : 32;
+        const uint32_t llo = rlo ? (uint32_t)__builtin_amdgcn_readlane((int)key, flo) : 0u;
+        const uint32_t lhi = rhi ? (uint32_t)__builtin_amdgcn_readlane((int)key, fhi) : 0u;
+        const uint32_t lead = upper ? lhi : llo;
+        const bool same = key != 0u && key == lead;
+        const unsigned long long sm = __ballot(same);
+        const uint32_t clo = (uint32_t)__popc((uint32_t)sm), chi = (uint32_t)__popc((uint32_t)(sm >> 32));
+        const uint32_t cnt = upper ? chi : clo;
+        const uint32_t tot = W ? half_sum_u32(same ? f : 0u) : cnt;
+        if (same) {
+            if (cnt >= 2u) {  // a singleton keeps its own insert
+                ik = lane == (upper ? fhi : flo) ? lead : 0u;
+                iff = tot;
+            }
+            key = 0u;
+        }
     }
+    if (ik) lds_insert(Kt, Ct, kHalfCap - 1u, ik, iff);
+}
+
+template <class H, bool W, bool G>
+__device__ __forceinline__ void half_body(const StepArgs& a, const int32_t* __restrict__ verts,
+                                          int32_t nv, uint32_t blk) {
+    __shared__ uint32_t s_key[4][2][kHalfCap];
+    __shared__ uint32_t s_cnt[4][2][kHalfCap];
+    const int lane = lane_id(), wv = threadIdx.x >> 6, hf = lane >> 5, hl = lane & 31;
+    const int32_t i0 = ((int32_t)blk * 4 + wv) * (2 * kHalfPairs);
+    if (i0 >= nv) return;  // wave-uniform
+    int32_t v[kHalfPairs], b[kHalfPairs], e[kHalfPairs], old[kHalfPairs];
+    bool run[kHalfPairs];
+    uint32_t key[kHalfPairs][2], fr[kHalfPairs][2];
+#pragma unroll
+    for (int p = 0; p < kHalfPairs; ++p) {
+        const int32_t idx = i0 + 2 * p + hf;
+        v[p] = verts[min(idx, nv - 1)];
+        run[p] = idx < nv;
+    }
+#pragma unroll
+    for (int p = 0; p < kHalfPairs; ++p) {
+        run[p] = run[p] && is_active<H>(a, v[p]);
+        b[p] = a.off[v[p]];
+        e[p] = a.off[v[p] + 1];
+        old[p] = a.keep[v[p]];
+    }
+    // every tier vertex has >= 1 edge, so e-1 is a valid index; a vertex not
+    // run loads its first edge only
+#pragma unroll
+    for (int p = 0; p < kHalfPairs; ++p)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int32_t j = b[p] + hl + 32 * i;
+            key[p][i] = H::template load_neighbor<W, G>(a, v[p], run[p] ? min(j, e[p] - 1) : b[p], &fr[p][i]);
+            if (!run[p] || j >= e[p]) key[p][i] = 0u;
+        }
+    uint32_t* K = s_key[wv][hf];
+    uint32_t* C = s_cnt[wv][hf];
+    uint32_t nch = 0u;
+#pragma unroll
+    for (int p = 0; p < kHalfPairs; ++p) {
+        if (__ballot(run[p]) == 0ull) continue;  // wave-uniform: both halves idle
+        for (uint32_t i = hl; i < kHalfCap; i += 32) {
+            K[i] = 0u;
+            C[i] = 0u;
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int i = 0; i < 2; ++i) half_insert<W>(K, C, key[p][i], fr[p][i]);
+        wave_lds_sync();
+        uint64_t best = 0ull;
+        for (uint32_t i = hl; i < kHalfCap; i += 32) {
+            const uint32_t kk = K[i];
+            if (kk) best = max_u64(best, H::label_score(a, v[p], kk, C[i]));
+        }
+        best = half_max_u64(best);
+        wave_lds_sync();  // the next pair clears what this scan read
+        bool changed = false;
+        if (hl == 0 && run[p]) {
+            const int32_t nl = packed_label(best);
+            H::update_vertex(a, v[p], nl, best);
+            changed = nl != old[p];
+            if (changed) note_changed(a, v[p]);
+        }
+        nch += (uint32_t)__popcll(__ballot(changed));
+        if (marking<H>(a) && shfl_i32(changed ? 1 : 0, lane & 32) != 0)
+            mark_vertex(a, v[p], b[p], e[p], hl, 32);
+    }
+    add_changes(a, nch);
 }
 
 // ------------------------------------------------------------------ wave ----
@@ -315,7 +448,7 @@ __device__ __forceinline__ void packed_body(const StepArgs& a, const int2* __res
 // (4 per lane), so one round of branch-free loads covers it: the loads of all
 // kWaveNV vertices are issued first, then each vertex is counted in the
 // wave's LDS table in turn.
-constexpr int kWaveNV = 2;
+constexpr int kWaveNV = 4;
 template <class H, bool W, bool G>
 __device__ __forceinline__ void wave_body(const StepArgs& a, const int32_t* __restrict__ verts,
                                           int32_t nv, uint32_t blk) {
@@ -348,6 +481,7 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int32_t* __re
         }
     uint32_t* K = s_key[wv];
     uint32_t* C = s_cnt[wv];
+    uint32_t nch = 0u;  // changed labels (lane 0)
 #pragma unroll
     for (int k = 0; k < kWaveNV; ++k) {
         if (!run[k]) continue;  // wave-uniform
@@ -374,10 +508,11 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int32_t* __re
             H::update_vertex(a, v[k], nl, best);
             changed = nl != old[k];
             if (changed) note_changed(a, v[k]);
+            nch += changed ? 1u : 0u;
         }
         if (marking<H>(a) && shfl_i32(changed ? 1 : 0, 0) != 0) mark_vertex(a, v[k], b[k], e[k], lane, 64);
-        count_changes(a, changed);
     }
+    add_changes(a, nch);
 }
 
 // ----------------------------------------------------------------- block ----
@@ -780,6 +915,11 @@ __global__ __launch_bounds__(256) void k_packed(StepArgs a, const int2* __restri
     packed_body<H, W, G>(a, groups, ngroups, nge, blockIdx.x);
 }
 template <class H, bool W, bool G>
+__global__ __launch_bounds__(256) void k_half(StepArgs a, const int32_t* __restrict__ verts,
+                                              int32_t nv) {
+    half_body<H, W, G>(a, verts, nv, blockIdx.x);
+}
+template <class H, bool W, bool G>
 __global__ __launch_bounds__(256) void k_wave(StepArgs a, const int32_t* __restrict__ verts,
                                               int32_t nv) {
     wave_body<H, W, G>(a, verts, nv, blockIdx.x);
@@ -834,6 +974,8 @@ __global__ __launch_bounds__(NT) void k_multi(const JobDesc* __restrict__ jobs,
     const JobDesc& jd = jobs[j];
     if constexpr (L == kLaunchPacked) {
         packed_body<H, W, G>(jd.a, jd.groups, jd.ngroups, jd.ngroups_e, blk);
+    } else if constexpr (L == kLaunchHalf) {
+        half_body<H, W, G>(jd.a, jd.half, jd.nhalf, blk);
     } else if constexpr (L == kLaunchWave) {
         wave_body<H, W, G>(jd.a, jd.wave, jd.nwave, blk);
     } else if constexpr (L >= kLaunchBlock0 && L < kLaunchBlock0 + kMaxBlockClasses) {
@@ -875,6 +1017,9 @@ static int launch_multi_t(const JobDesc* jobs, const int32_t* cum, int32_t nj, c
     if (grid[kLaunchPacked])
         hipLaunchKernelGGL((k_multi<H, W, false, kLaunchPacked, 256>), dim3(grid[kLaunchPacked]),
                            dim3(256), 0, s, jobs, row(kLaunchPacked), nj, 0u);
+    if (grid[kLaunchHalf])
+        hipLaunchKernelGGL((k_multi<H, W, false, kLaunchHalf, 256>), dim3(grid[kLaunchHalf]),
+                           dim3(256), 0, s, jobs, row(kLaunchHalf), nj, 0u);
     if (grid[kLaunchWave])
         hipLaunchKernelGGL((k_multi<H, W, false, kLaunchWave, 256>), dim3(grid[kLaunchWave]),
                            dim3(256), 0, s, jobs, row(kLaunchWave), nj, 0u);
@@ -917,6 +1062,7 @@ int launch_multi(bool weighted, const JobDesc* jobs, const int32_t* cum, int32_t
 void job_blocks(const Plan& p, int32_t* blocks) {
     for (int L = 0; L < kLaunches; ++L) blocks[L] = 0;
     blocks[kLaunchPacked] = (int32_t)((p.groups.size() + 4 * kPackNG - 1) / (4 * kPackNG));
+    blocks[kLaunchHalf] = (int32_t)((p.half.size() + 8 * kHalfPairs - 1) / (8 * kHalfPairs));
     blocks[kLaunchWave] = (int32_t)((p.wave.size() + 4 * kWaveNV - 1) / (4 * kWaveNV));
     for (int c = 0; c < p.nclasses; ++c) blocks[kLaunchBlock0 + c] = (int32_t)p.block[c].size();
     blocks[kLaunchHubAgg] = (int32_t)p.hub_chunks.size();
@@ -1055,6 +1201,11 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
     }
     mark(GLP_TIER_PACKED, 1);
     mark(GLP_TIER_WAVE, 0);
+    if (!p.half.empty()) {
+        const int32_t nh = (int32_t)p.half.size();
+        hipLaunchKernelGGL((k_half<H, W, G>), dim3(grid_for(nh, 8 * kHalfPairs)), dim3(256), 0, s, a,
+                           g.d_half, nh);
+    }
     if (!p.wave.empty()) {
         const int32_t nw = (int32_t)p.wave.size();
         hipLaunchKernelGGL((k_wave<H, W, G>), dim3(grid_for(nw, 4 * kWaveNV)), dim3(256), 0, s, a,

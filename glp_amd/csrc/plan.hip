@@ -25,7 +25,8 @@ struct TuningField {
     int32_t Tuning::*field;
 };
 const TuningField kFields[] = {
-    {"pack_max_deg", &Tuning::pack_max_deg}, {"wave_max_deg", &Tuning::wave_max_deg},
+    {"pack_max_deg", &Tuning::pack_max_deg}, {"half_max_deg", &Tuning::half_max_deg},
+    {"wave_max_deg", &Tuning::wave_max_deg},
     {"block_max_deg", &Tuning::block_max_deg}, {"hub_chunk", &Tuning::hub_chunk},
     {"part_keys", &Tuning::part_keys},       {"part_limit", &Tuning::part_limit},
     {"block_threads", &Tuning::block_threads},
@@ -73,6 +74,8 @@ Tuning& default_tuning() {
 
 static int check_tuning(const Tuning& t) {
     if (t.pack_max_deg < 0 || t.pack_max_deg > 64) return set_error(GLP_E_ARG, "pack_max_deg must be in [0,64]");
+    if (t.half_max_deg < 0 || t.half_max_deg > kHalfMaxDeg)
+        return set_error(GLP_E_ARG, "half_max_deg must be in [0, %d]", kHalfMaxDeg);
     if (t.wave_max_deg < t.pack_max_deg || t.wave_max_deg > kWaveCap / 2)
         return set_error(GLP_E_ARG, "wave_max_deg must be in [pack_max_deg, %d]", kWaveCap / 2);
     // the largest LDS table (2*deg slots of 8 bytes) must fit a CU's 160 KB
@@ -132,7 +135,7 @@ int build_plan(const HostCSR& g, const Tuning& t, Plan& p) {
         }
         close_group();
         if (d <= t.wave_max_deg) {
-            p.wave.push_back(v);
+            (d <= t.half_max_deg ? p.half : p.wave).push_back(v);
             p.edges[1] += d;
             p.verts[1]++;
         } else if (d <= t.block_max_deg) {

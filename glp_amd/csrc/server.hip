@@ -185,6 +185,8 @@ int glp_server::tick() {
         d.groups = dg.d_groups;
         d.ngroups = (int32_t)p.groups.size();
         d.ngroups_e = (int32_t)p.groups_with_edges;
+        d.half = dg.d_half;
+        d.nhalf = (int32_t)p.half.size();
         d.wave = dg.d_wave;
         d.nwave = (int32_t)p.wave.size();
         for (int c = 0; c < kMaxBlockClasses; ++c) d.block[c] = dg.d_block[c];

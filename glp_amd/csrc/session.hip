@@ -28,6 +28,7 @@ DeviceGraph::~DeviceGraph() {
         (void)hipFree(w);
     }
     (void)hipFree(d_groups);
+    (void)hipFree(d_half);
     (void)hipFree(d_wave);
     for (int c = 0; c < kMaxBlockClasses; ++c) (void)hipFree(d_block[c]);
     (void)hipFree(d_hubs);
@@ -50,7 +51,7 @@ static int upload(T** dst, const T* src, size_t count, hipStream_t s, size_t& by
 size_t device_graph_bytes(const HostCSR& g, const Plan& p) {
     size_t b = sizeof(int32_t) * ((size_t)g.n + 1) + sizeof(int32_t) * (size_t)g.m;
     if (g.weighted()) b += sizeof(uint32_t) * (size_t)g.m;
-    b += sizeof(int2) * p.groups.size() + sizeof(int32_t) * p.wave.size();
+    b += sizeof(int2) * p.groups.size() + sizeof(int32_t) * (p.half.size() + p.wave.size());
     for (int c = 0; c < p.nclasses; ++c) b += sizeof(int32_t) * p.block[c].size();
     b += sizeof(int32_t) * p.hubs.size() + sizeof(int2) * p.hub_chunks.size();
     b += (sizeof(int64_t) * 2 + sizeof(uint32_t)) * p.hubs.size();
@@ -94,6 +95,7 @@ int device_plan_upload(DeviceGraph* dg, hipStream_t s) {
     size_t bytes = 0;
     const Plan& p = dg->plan;
     if ((rc = upload(&dg->d_groups, p.groups.data(), p.groups.size(), s, bytes))) return rc;
+    if ((rc = upload(&dg->d_half, p.half.data(), p.half.size(), s, bytes))) return rc;
     if ((rc = upload(&dg->d_wave, p.wave.data(), p.wave.size(), s, bytes))) return rc;
     for (int c = 0; c < p.nclasses; ++c)
         if ((rc = upload(&dg->d_block[c], p.block[c].data(), p.block[c].size(), s, bytes))) return rc;

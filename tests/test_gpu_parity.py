@@ -84,6 +84,28 @@ def test_classic_tier_boundaries():
     assert_steps_equal(gpu_steps(g, 5), O.classic(off, nbr, 5))
 
 
+@pytest.mark.parametrize("half,pack", [(0, 32), (48, 32), (64, 32), (64, 0)])
+def test_half_wave_tier(half, pack):
+    """Degrees in (pack_max_deg, half_max_deg] run two per wavefront (one per
+    32-lane half); half_max_deg=0 sends them through the one-vertex wave path.
+    pack_max_deg=0 puts degrees 1..64 in the half tier.  Classic unweighted and
+    weighted, and layered, must match the oracle either way."""
+    with G.tuning(pack_max_deg=pack, half_max_deg=half):
+        g = G.Graph.rmat(14, 16, seed=33)
+        off, nbr, _ = g.csr()
+        _, dump, timing = G.run(g, 6, dump=True)
+        assert_steps_equal(dump, O.classic(off, nbr, 6))
+        gw = G.Graph.rmat(13, 16, seed=34, wmax=8)
+        offw, nbrw, w = gw.csr()
+        _, dumpw, _ = G.run(gw, 5, dump=True, weighted=True)
+        assert_steps_equal(dumpw, O.classic(offw, nbrw, 5, w=w))
+        _, duml, _ = G.run(gw, 4, dump=True, method=G.LAYERED, gamma=(1, 4), weighted=True)
+        assert_steps_equal(duml, O.layered(offw, nbrw, 4, 1, 4, w=w)[0])
+        n, edges = tier_boundary_graph(seed=3)
+        gb, offb, nbrb, _ = hand_graph(edges, n)
+        assert_steps_equal(gpu_steps(gb, 4), O.classic(offb, nbrb, 4))
+
+
 @pytest.mark.parametrize("limit", [0, 8])
 def test_hub_partition_and_halving(limit):
     """Tiny thresholds push most vertices through the hub passes; part_limit=8
