@@ -43,23 +43,29 @@ def tier_bytes(edges, verts, weighted=False):
     return (12 if weighted else 8) * edges + 12 * verts
 
 
-def cpu_baseline(off, nbr, target_s, threads, max_steps):
+def cpu_baseline(off, nbr, target_s, threads):
     """The reference CPU loop (tutorial/CPUlp.cuh:28-92 sync_run_1, restated:
-    OpenMP over vertices, a std::map count per vertex, live label reads) run
-    for whole steps over the same graph from L0, as many as fit in target_s
-    (at least one, at most max_steps)."""
+    OpenMP over vertices, a std::map count per vertex, live label reads),
+    timed over a strided sample of the same graph's vertices from L0: every
+    k-th vertex, k chosen from a 1/256 pilot so the timed sample takes about
+    target_s.  The rate is the sample's edges over its time."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    m = int(off[-1])
-    _, secs1 = O.cpulp_run(off, nbr, 1, threads, 0)
-    steps = int(max(1, min(max_steps, target_s // max(secs1, 1e-9))))
-    secs = secs1
-    if steps > 1:
-        _, secs = O.cpulp_run(off, nbr, steps, threads, 0)
-    return {"value": steps * m / secs / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
-            "sample": "the first %d step(s) of verbatim CPUlp on the same graph (all %d vertices, "
-                      "%d edges per step), %.2f s; a separate first step took %.2f s"
-                      % (steps, len(off) - 1, m, secs, secs1)}
+    n = len(off) - 1
+    deg = np.diff(off.astype(np.int64))
+    pilot = np.arange(0, n, 256, dtype=np.int32)
+    lab = np.arange(n, dtype=np.int32)
+    secs_p = O.cpulp_sample(off, nbr, lab, pilot, threads)
+    rate = max(int(deg[pilot].sum()), 1) / max(secs_p, 1e-6)
+    stride = int(min(256, max(1, round(int(deg.sum()) / (rate * target_s)))))
+    verts = np.arange(0, n, stride, dtype=np.int32)
+    lab = np.arange(n, dtype=np.int32)
+    secs = O.cpulp_sample(off, nbr, lab, verts, threads)
+    e = int(deg[verts].sum())
+    return {"value": e / secs / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
+            "sample": "one verbatim CPUlp step from L0 over every %d-th vertex (%d vertices, %d "
+                      "edges of %d) of the same graph, %.2f s on %d threads"
+                      % (stride, len(verts), e, int(deg.sum()), secs, threads)}
 
 
 def bench_batched(args, G):
@@ -366,7 +372,7 @@ def main():
     if not args.no_cpu:
         log("cpu baseline (%d threads, ~%.0f s)" % (threads, args.cpu_seconds))
         off, nbr, _ = g.csr()
-        cpu = cpu_baseline(off, nbr, args.cpu_seconds, threads, args.steps)
+        cpu = cpu_baseline(off, nbr, args.cpu_seconds, threads)
         log("cpu baseline %.4f GTEPS" % cpu["value"])
 
     out = {

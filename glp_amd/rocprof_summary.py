@@ -3,7 +3,8 @@
     python -m glp_amd.rocprof_summary stats DIR --command "..." > profiles/X.txt
     python -m glp_amd.rocprof_summary calibrate FETCH_DIR WRITE_DIR --bytes N --command "..."
     python -m glp_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR --calibration FILE
-            --warmup W --steps K [--gather-steps G] --command "..."
+            --warmup W --steps K [--gather-steps G] --workload KEY --command "..."
+    python -m glp_amd.rocprof_summary sizes DIR --warmup W --steps K [--gather-steps G] --command "..."
 
 `stats` condenses a --kernel-trace --stats run.  `calibrate` compares the
 FETCH_SIZE / WRITE_SIZE that rocprofv3 reports for device-to-device copies of
@@ -149,6 +150,37 @@ def cmd_traffic(a):
     print("\n".join(out + ["workload %s" % a.workload] + lines))
 
 
+SIZE_COUNTERS = ("TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum",
+                 "TCC_BUBBLE_sum")
+
+
+def cmd_sizes(a):
+    """Read bytes per step from the L2 request-size counters (32/64/128 B),
+    next to TCC_BUBBLE (the 128 B count FETCH_SIZE's formula uses)."""
+    per = defaultdict(lambda: defaultdict(float))
+    for c in SIZE_COUNTERS:
+        steps = split_steps(read_counter(a.dir, c))
+        groups = {"main": steps[a.warmup:a.warmup + a.steps]}
+        if a.gather_steps:
+            g0 = a.warmup + a.steps + 1
+            groups["gather"] = steps[g0:g0 + a.gather_steps]
+        for gname, ss in groups.items():
+            for st in ss:
+                for _, fam, v in st:
+                    per[(gname, fam)][c] += v / len(ss)
+    out = ["# L2-to-memory read requests by size per LP step (rocprofv3 TCC_EA0_RDREQ_{32B,64B,128B},",
+           "# TCC_BUBBLE), averaged over the timed steps, and the read bytes they imply",
+           "# command: %s" % a.command, "#",
+           "# session kernel            req_32B      req_64B     req_128B  TCC_BUBBLE    read_bytes"]
+    for (gname, fam), v in sorted(per.items()):
+        b = 32 * v[SIZE_COUNTERS[0]] + 64 * v[SIZE_COUNTERS[1]] + 128 * v[SIZE_COUNTERS[2]]
+        if b < 1e6:
+            continue
+        out.append("# %-7s %-16s %10.0f %12.0f %12.0f %11.0f %13.0f" % (
+            gname, fam, v[SIZE_COUNTERS[0]], v[SIZE_COUNTERS[1]], v[SIZE_COUNTERS[2]], v[SIZE_COUNTERS[3]], b))
+    print("\n".join(out))
+
+
 def traffic_for(path, workload, session, families):
     """Sum of the per-step traffic of the given kernel families measured on
     this workload, or None."""
@@ -186,8 +218,14 @@ def main():
     t.add_argument("--gather-steps", type=int, default=0)
     t.add_argument("--workload", required=True, help="key, e.g. rmat-s25-ef16-seed1-classic")
     t.add_argument("--command", required=True)
+    z = sub.add_parser("sizes")
+    z.add_argument("dir")
+    z.add_argument("--warmup", type=int, required=True)
+    z.add_argument("--steps", type=int, required=True)
+    z.add_argument("--gather-steps", type=int, default=0)
+    z.add_argument("--command", required=True)
     a = ap.parse_args()
-    {"stats": cmd_stats, "calibrate": cmd_calibrate, "traffic": cmd_traffic}[a.cmd](a)
+    {"stats": cmd_stats, "calibrate": cmd_calibrate, "traffic": cmd_traffic, "sizes": cmd_sizes}[a.cmd](a)
 
 
 if __name__ == "__main__":
