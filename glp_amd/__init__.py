@@ -96,6 +96,7 @@ _sig("glp_graph_save_bin", ctypes.c_int, _vp, ctypes.c_char_p)
 _sig("glp_graph_degree_sort", ctypes.c_int, _vp, _pp, _c_i32p)
 _sig("glp_tool_edgelist2bin", ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p)
 _sig("glp_tool_normalize", ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p)
+_sig("glp_tool_direct_edgelist2bin", ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32)
 _sig("glp_gen_rmat", ctypes.c_int, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
      ctypes.c_double, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _pp)
 _sig("glp_gen_planted", ctypes.c_int, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
@@ -368,6 +369,17 @@ def edgelist2bin(txt_path, out_path):
 
 def normalize(bin_path, out_path):
     _check(_lib.glp_tool_normalize(os.fsencode(bin_path), os.fsencode(out_path)))
+
+
+# the id cut-off tools/direct_edgelist2bin.cpp:137 hard-codes
+DIRECT_ID_CUTOFF = 5013066
+
+
+def direct_edgelist2bin(bin_path, out_path, id_cutoff=DIRECT_ID_CUTOFF):
+    """Directed .bin writer (tools/direct_edgelist2bin.cpp); id_cutoff <= 0
+    keeps every edge."""
+    _check(_lib.glp_tool_direct_edgelist2bin(os.fsencode(bin_path), os.fsencode(out_path),
+                                             int(id_cutoff)))
 
 
 class Session:

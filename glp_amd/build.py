@@ -35,6 +35,7 @@ PROGRAMS = {
     "GPULP": ["gpulp_main.cpp"],
     "edgelist2bin": ["tool_edgelist2bin.cpp"],
     "normalize": ["tool_normalize.cpp"],
+    "direct_edgelist2bin": ["tool_direct_edgelist2bin.cpp"],
     "glp_autotune": ["autotune_main.cpp"],
 }
 

@@ -421,3 +421,54 @@ extern "C" int glp_tool_normalize(const char* bin_path, const char* out_path) {
     }
     return ofs ? GLP_OK : set_error(GLP_E_IO, "write failed on '%s'", out_path);
 }
+
+// tools/direct_edgelist2bin.cpp:113-205 (buildfrombin, the path its main
+// runs): reads .bin pairs (u,v), drops self-loops, keeps an edge only when
+// u or v is below id_cutoff (the reference hard-codes 5013066 at :137;
+// id_cutoff <= 0 keeps every edge), keeps u -> v only (directed, duplicates
+// kept), orders the vertices seen by non-increasing out-degree with the same
+// unstable std::sort on the ascending-id list, and writes each vertex's
+// pairs (x, y) with its neighbours in ascending original id.
+extern "C" int glp_tool_direct_edgelist2bin(const char* bin_path, const char* out_path,
+                                           int32_t id_cutoff) {
+    if (!bin_path || !out_path) return set_error(GLP_E_ARG, "null argument");
+    std::vector<char> buf;
+    if (!read_file(bin_path, buf)) return set_error(GLP_E_IO, "cannot read '%s'", bin_path);
+    const int32_t* rec = reinterpret_cast<const int32_t*>(buf.data());
+    const size_t nrec = buf.size() / 8;
+    std::vector<std::vector<int>> adj;
+    std::set<int> vset;
+    for (size_t i = 0; i < nrec; ++i) {
+        const int u = rec[2 * i], v = rec[2 * i + 1];
+        if (u == v) continue;
+        if (id_cutoff > 0 && !(u < id_cutoff || v < id_cutoff)) continue;
+        if (u < 0 || v < 0) return set_error(GLP_E_FORMAT, "negative vertex id in '%s'", bin_path);
+        vset.insert(u);
+        vset.insert(v);
+        const size_t top = (size_t)std::max(u, v);
+        if (top >= adj.size()) adj.resize(top + 1);
+        adj[u].push_back(v);
+    }
+    std::vector<int> vlist(vset.begin(), vset.end());
+    std::vector<NodeDeg> nd(vlist.size());
+    for (size_t i = 0; i < vlist.size(); ++i) {
+        nd[i].node = vlist[i];
+        nd[i].degree = (int)adj[vlist[i]].size();
+    }
+    std::sort(nd.begin(), nd.end(), ByDegreeDesc());
+    for (size_t i = 0; i < nd.size(); ++i) vlist[i] = nd[i].node;
+    std::map<int, int> map;
+    for (size_t i = 0; i < vlist.size(); ++i) map[vlist[i]] = (int)i;
+    std::ofstream ofs(out_path, std::ios::binary | std::ios::trunc);
+    if (!ofs) return set_error(GLP_E_IO, "cannot write '%s'", out_path);
+    for (size_t i = 0; i < vlist.size(); ++i) {
+        std::vector<int>& nb = adj[vlist[i]];
+        std::sort(nb.begin(), nb.end());
+        const int x = map[vlist[i]];
+        for (int yv : nb) {
+            const int32_t w[2] = {x, map[yv]};
+            ofs.write(reinterpret_cast<const char*>(w), sizeof w);
+        }
+    }
+    return ofs ? GLP_OK : set_error(GLP_E_IO, "write failed on '%s'", out_path);
+}

@@ -27,6 +27,17 @@ int glp_tool_edgelist2bin(const char* txt_path, const char* out_path);
  * (x,y) x<y pairs.  The reference writes "sorted_<basename>.bin".            */
 int glp_tool_normalize(const char* bin_path, const char* out_path);
 
+/* Replaces tools/direct_edgelist2bin.cpp:113-205 (buildfrombin, the path
+ * its main runs): reads .bin pairs (u,v), drops self-loops, keeps an edge
+ * only if u or v is below id_cutoff (the reference hard-codes 5013066 at
+ * :137; id_cutoff <= 0 keeps every edge), keeps u -> v only (directed,
+ * duplicates kept), orders the vertices seen by non-increasing out-degree
+ * with the same unstable std::sort and comparator on the ascending-id list,
+ * and writes each vertex's (x,y) pairs with neighbours ascending by original
+ * id.  The reference writes "sorted_direct_<basename>.bin".                  */
+int glp_tool_direct_edgelist2bin(const char* bin_path, const char* out_path,
+                                 int32_t id_cutoff);
+
 /* Writes a graph back as a .bin: for every vertex u in id order, each
  * neighbour v > u in CSR order as an (u,v) pair (the inverse of
  * glp_graph_load_bin for undirected graphs without duplicates); weighted

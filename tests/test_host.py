@@ -159,6 +159,50 @@ def test_normalize_matches_reference(tmp_path):
     assert out.read_bytes() == ref
 
 
+def test_direct_edgelist2bin_hand_case(tmp_path):
+    """Directed writer: self-loops dropped, duplicates kept, u -> v only,
+    vertices by non-increasing out-degree (a short list sorts stably),
+    neighbours ascending by original id; the id cut-off keeps an edge only
+    when an endpoint is below it."""
+    src = tmp_path / "d.bin"
+    write_pairs(str(src), [(0, 1), (0, 2), (2, 1), (1, 1), (3, 0), (0, 1)])
+    out = tmp_path / "o.bin"
+    G.direct_edgelist2bin(str(src), str(out))
+    got = np.fromfile(str(out), dtype=np.int32).reshape(-1, 2).tolist()
+    # order 0 (3 out-edges), 2 (1), 3 (1), 1 (0): new ids 0, 1, 2, 3
+    assert got == [[0, 3], [0, 3], [0, 1], [1, 3], [2, 0]]
+    G.direct_edgelist2bin(str(src), str(out), id_cutoff=1)   # drops (2,1)
+    got = np.fromfile(str(out), dtype=np.int32).reshape(-1, 2).tolist()
+    # order 0 (3), 3 (1), 1 (0), 2 (0): new ids 0, 1, 2, 3
+    assert got == [[0, 2], [0, 2], [0, 3], [1, 0]]
+    g = G.Graph.load_bin(str(out), directed=True)
+    off, nbr, _ = g.csr()
+    assert off.tolist() == [0, 3, 4, 4, 4] and nbr.tolist() == [2, 2, 3, 0]
+
+
+@needs_ref
+def test_direct_edgelist2bin_matches_reference(tmp_path):
+    src = tmp_path / "g.bin"
+    rng = np.random.default_rng(8)
+    pairs = [tuple(x) for x in rng.integers(0, 300, size=(3000, 2))]
+    cut = 5013066
+    pairs += [(cut + 1, cut + 2), (cut - 1, cut + 3), (cut + 4, 5), (7, 7), (12, 12)]
+    pairs += pairs[:200]                                    # duplicates
+    write_pairs(str(src), pairs)
+    subprocess.run([os.path.join(REF, "direct_edgelist2bin"), str(src)], cwd=tmp_path, check=True,
+                   capture_output=True)
+    ref = (tmp_path / "sorted_direct_g.bin").read_bytes()
+    out = tmp_path / "mine.bin"
+    G.direct_edgelist2bin(str(src), str(out))
+    assert out.read_bytes() == ref
+    # the CLI program writes the reference's output name
+    sub = tmp_path / "cli"
+    sub.mkdir()
+    subprocess.run([os.path.join(ROOT, "glp_amd", "bin", "direct_edgelist2bin"), str(src)], cwd=sub,
+                   check=True, capture_output=True)
+    assert (sub / "sorted_direct_g.bin").read_bytes() == ref
+
+
 def test_generators_deterministic_and_thread_independent():
     a = G.Graph.rmat(12, 16, seed=5, threads=1).csr()
     b = G.Graph.rmat(12, 16, seed=5, threads=4).csr()
