@@ -23,9 +23,9 @@ if not os.path.exists(LIB_PATH):
 
 _lib = ctypes.CDLL(LIB_PATH)
 
-CLASSIC, LAYERED, SLPA, USER, COUNTMIN, FREQUENT, BUFFER = 1, 2, 3, 4, 5, 6, 7
+CLASSIC, LAYERED, SLPA, USER, COUNTMIN, FREQUENT, BUFFER, SEEDED = 1, 2, 3, 4, 5, 6, 7, 8
 METHOD_NAMES = {CLASSIC: "classic", LAYERED: "layered", SLPA: "slpa", USER: "user",
-                COUNTMIN: "countmin", FREQUENT: "frequent", BUFFER: "buffer"}
+                COUNTMIN: "countmin", FREQUENT: "frequent", BUFFER: "buffer", SEEDED: "seeded"}
 TIER_NAMES = ["packed", "wave", "block", "hub", "gather", "pick", "aux", "spare"]
 
 _c_i32p = ctypes.POINTER(ctypes.c_int32)
@@ -153,6 +153,9 @@ def lib():
 _sig("glp_autotune", ctypes.c_int, _vp, ctypes.POINTER(Params), ctypes.c_int32, ctypes.c_char_p,
      ctypes.POINTER(ctypes.c_double))
 _sig("glp_session_set_gamma", ctypes.c_int, _vp, ctypes.c_int32, ctypes.c_int32)
+_sig("glp_session_set_seeds", ctypes.c_int, _vp, ctypes.c_void_p, ctypes.c_int32)
+_sig("glp_run_seeded", ctypes.c_int, _vp, ctypes.POINTER(Params), ctypes.c_void_p, ctypes.c_int32,
+     _c_i32p, _c_i32p, ctypes.POINTER(Timing))
 
 
 def autotune(graph, params, steps=5, out_path=None):
@@ -422,6 +425,12 @@ class Session:
     def set_gamma(self, num, den):
         _check(_lib.glp_session_set_gamma(self._h, num, den))
 
+    def set_seeds(self, seeds):
+        """Seeded LP: seeds is a length-n mask (nonzero = seed); resets the
+        labels to v on seeds and -1 elsewhere."""
+        m = np.ascontiguousarray(np.asarray(seeds) != 0, dtype=np.uint8)
+        _check(_lib.glp_session_set_seeds(self._h, m.ctypes.data if len(m) else None, len(m)))
+
     def labels(self):
         out = np.zeros(max(self.n, 1), dtype=np.int32)
         _check(_lib.glp_session_labels(self._h, _i32(out)))
@@ -467,16 +476,26 @@ class Session:
         return dom[:self.n], ns[:self.n], sets[:self.n * kmax].reshape(self.n, kmax)
 
 
-def run(graph, steps, params=None, dump=False, **kw):
-    """Runs `steps` LP steps from L0[v]=v.  Returns labels (and the per-step
+def run(graph, steps, params=None, dump=False, seeds=None, **kw):
+    """Runs `steps` LP steps from L0[v]=v (seeded LP: from v on seeds, -1
+    elsewhere; `seeds` is a length-n mask).  Returns labels (and the per-step
     label dump when dump=True) and the timing record."""
     p = params if params is not None else make_params(**kw)
     n = graph.n
     labels = np.zeros(max(n, 1), dtype=np.int32)
     dump_arr = np.zeros(max(steps * n, 1), dtype=np.int32) if dump else None
     t = Timing()
-    _check(_lib.glp_run(graph.handle, ctypes.byref(p), steps, _i32(labels),
-                        _i32(dump_arr) if dump else None, ctypes.byref(t)))
+    if p.method == SEEDED:
+        if seeds is None:
+            raise GLPError(-1, "seeded LP needs seeds")
+        m = np.ascontiguousarray(np.asarray(seeds) != 0, dtype=np.uint8)
+        if len(m) != n:
+            raise GLPError(-1, "seeds hold %d entries, the graph has %d vertices" % (len(m), n))
+        _check(_lib.glp_run_seeded(graph.handle, ctypes.byref(p), m.ctypes.data if n else None, steps,
+                                   _i32(labels), _i32(dump_arr) if dump else None, ctypes.byref(t)))
+    else:
+        _check(_lib.glp_run(graph.handle, ctypes.byref(p), steps, _i32(labels),
+                            _i32(dump_arr) if dump else None, ctypes.byref(t)))
     labels = labels[:n]
     if dump:
         return labels, dump_arr[:steps * n].reshape(steps, n), t.as_dict()

@@ -49,6 +49,8 @@ extern "C" int glp_batch_create(int32_t ngraphs, glp_graph* const* graphs, const
     if (p->method == GLP_SLPA || p->method == GLP_COUNTMIN)
         return set_error(GLP_E_UNSUPPORTED,
                          "batched SLPA / count-min would change results (stream and hash depend on ids)");
+    if (p->method == GLP_SEEDED)
+        return set_error(GLP_E_UNSUPPORTED, "seeded LP has per-graph seeds: run each graph on its own");
     auto b = std::make_unique<glp_batch>();
     b->p = *p;
     b->graphs.assign(graphs, graphs + ngraphs);

@@ -146,7 +146,7 @@ struct JobDesc {
     int32_t blocks[kLaunches];
 };
 
-enum class Method { Classic, Layered, Slpa, User };
+enum class Method { Classic, Layered, Slpa, User, Seeded };
 
 // One-time kernel attribute set-up (LDS limits); call before any capture.
 void init_kernels();
@@ -159,6 +159,8 @@ int launch_pick(Method method, const StepArgs& a, int32_t* shown, hipStream_t s)
 int launch_gather(const StepArgs& a, int64_t m, int32_t* adj, hipStream_t s);
 int launch_fill_i32(int32_t* p, int32_t value, int64_t count, hipStream_t s);
 int launch_iota(int32_t* p, int64_t count, hipStream_t s);
+// p[i] = seeds[i] ? i : -1 (seeded LP's initial labels).
+int launch_seed_labels(int32_t* p, const uint8_t* seeds, int64_t count, hipStream_t s);
 int launch_end_step(unsigned long long* acc, unsigned long long* ring, int32_t* step_ctr,
                     int32_t ring_size, hipStream_t s, int32_t* fctl = nullptr);
 // Fills count bytes (count a multiple of 4, p 4-byte aligned) with zero.

@@ -18,6 +18,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 #include "device_prims.h"
 
@@ -54,6 +55,8 @@ struct StepArgs {
     // labels) the gather rewrites only the edges whose source bit is set
     const uint32_t* chg_r;
     uint32_t* chg_w;
+    // seeded LP: seeds[v] != 0 marks a seed (its label stays v)
+    const uint8_t* seeds;
 };
 
 __host__ __device__ __forceinline__ uint32_t order_i32(int32_t s) {
@@ -162,6 +165,39 @@ struct SlpaHooks {
         a.Lw[vid] = label;
     }
 };
+
+// ---- seeded (semi-supervised) LP (prop/layeredLprop.cuh:161-180,222-224;
+// header/kernel.cuh:453-466 and the *_seed tier kernels): seeds start with
+// and keep their own id, every other vertex starts unlabeled (-1).  A
+// non-seed takes the classic argmax over its labeled neighbours only (an
+// unlabeled neighbour shows key 0, which no tier counts) and keeps its
+// label when no neighbour is labeled.
+struct SeededHooks {
+    static constexpr bool kPick = false;
+    static constexpr bool kSkipIsolated = true;  // isolated vertices keep their label
+    static constexpr bool kFrontier = false;
+    static constexpr bool kSeeded = true;        // seeds are never recomputed
+    template <bool W, bool G>
+    __device__ static __forceinline__ uint32_t load_neighbor(const StepArgs& a, int32_t vid,
+                                                             int32_t j, uint32_t* freq) {
+        return ClassicHooks::load_neighbor<W, G>(a, vid, j, freq);  // label -1 gives key 0
+    }
+    __device__ static __forceinline__ uint64_t label_score(const StepArgs& a, int32_t vid,
+                                                           uint32_t key, uint32_t freq) {
+        return ClassicHooks::label_score(a, vid, key, freq);
+    }
+    __device__ static __forceinline__ void update_vertex(const StepArgs& a, int32_t vid,
+                                                         int32_t label, uint64_t score) {
+        (void)score;
+        a.Lw[vid] = label;
+    }
+};
+
+// Hook sets without a kSeeded member have no fixed vertices.
+template <class H, class = void>
+struct seeded_of : std::false_type {};
+template <class H>
+struct seeded_of<H, std::void_t<decltype(H::kSeeded)>> : std::bool_constant<H::kSeeded> {};
 
 }  // namespace glp
 

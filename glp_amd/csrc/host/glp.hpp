@@ -15,7 +15,9 @@
 
 #include <glp.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -129,6 +131,11 @@ public:
         return r;
     }
     glp_session* session() { return s; }
+    // Seeded LP: sets the seed mask (n bytes) and resets the labels.
+    void set_seeds(const std::vector<uint8_t>& seeds) {
+        open();
+        check(glp_session_set_seeds(s, seeds.data(), (int32_t)seeds.size()), "seeds");
+    }
 
 protected:
     void open() {
@@ -200,6 +207,42 @@ public:
         sets.resize((size_t)G->n * kmax);
         check(glp_session_slpa_post(s, kmax, dominant.data(), nsets.data(), sets.data()), "post");
     }
+};
+
+// Seed file as layeredLprop::assign_seeds_for_device reads it
+// (prop/layeredLprop.cuh:161-180): one vertex id per line.  Ids outside
+// [0, n) are rejected (the reference writes out of bounds).
+inline std::vector<uint8_t> read_seed_file(const std::string& path, int32_t n) {
+    std::ifstream ifs(path);
+    if (!ifs) throw Error(GLP_E_IO, "cannot read seed file '" + path + "'");
+    std::vector<uint8_t> seeds((size_t)std::max(n, 0), 0);
+    std::string line;
+    while (std::getline(ifs, line)) {
+        if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+        const long id = std::stol(line);
+        if (id < 0 || id >= n) throw Error(GLP_E_ARG, "seed id " + line + " outside the graph");
+        seeds[(size_t)id] = 1;
+    }
+    return seeds;
+}
+
+// Seeded LP as prop/layeredLprop.cuh runs it (despite its name that class is
+// the seed-constrained propagator; main.cu includes it without a method
+// number): seeds from a seed file, L_0 = v on seeds and -1 elsewhere
+// (header/kernel.cuh:453-466), run(niter) executes niter-1 steps like the
+// loop at prop/layeredLprop.cuh:388.  Returns seconds.
+class layeredLprop : public Propagator {
+public:
+    layeredLprop(CSRGraph* G, const std::string& seedfile, bool weighted = false)
+        : Propagator(G, [&] { auto p = make_params(GLP_SEEDED); p.weighted = weighted; return p; }()),
+          seeds(read_seed_file(seedfile, G->n)) {}
+    double run(int niter) {
+        set_seeds(seeds);
+        check(glp_session_step(s, niter > 1 ? niter - 1 : 0), "step");
+        check(glp_session_sync(s), "sync");
+        return timing().steps_ms / 1e3;
+    }
+    std::vector<uint8_t> seeds;
 };
 
 // Method 4: the compiled-in user hooks (glp_amd/csrc/user_hooks.h).

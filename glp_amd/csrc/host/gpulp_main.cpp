@@ -5,7 +5,8 @@
 //   GPULP method=1 iter=20 input=x.bin output=out.log ifdirect=0 ifweighted=0
 //
 // method: 1 classic, 2 layered, 3 SLPA, 4 user hooks (main.cu:63-101), and
-// this library's counting methods 5 count-min, 6 frequent, 7 buffer.  As in
+// this library's counting methods 5 count-min, 6 frequent, 7 buffer, and
+// 8 seeded LP (prop/layeredLprop.cuh; needs seeds=PATH).  As in
 // the reference, niter iterations execute niter-1 propagation steps
 // (prop/classicLprop.cuh:361) and one result line is appended to the output
 // file in main.cu:106-121's format.  Extra keys (extensions):
@@ -14,6 +15,7 @@
 //   seed=S r=p/q   SLPA stream seed and threshold
 //   cm=d,W[,a]     count-min rows, width, aliased rows (method 5)
 //   cap=C          frequent / buffer table capacity (methods 6, 7)
+//   seeds=PATH     seed file, one vertex id per line (method 8)
 //   gather=1       separate gather pass;  replay=1 HIP-graph step replay
 //   labels=PATH    write the final labels as little-endian int32
 //   tuning=PATH    key=value tuning file;  autotune=1 tune on this graph first
@@ -34,7 +36,8 @@ static void usage() {
     std::printf(
         "usage: GPULP method niter input.bin output.log ifDirected [ifWeighted] [key=value ...]\n"
         "       GPULP method=1 iter=20 input=g.bin output=out.log ifdirect=0 ifweighted=0\n"
-        "methods: 1 classic, 2 layered, 3 SLPA, 4 user hooks, 5 count-min, 6 frequent, 7 buffer\n");
+        "methods: 1 classic, 2 layered, 3 SLPA, 4 user hooks, 5 count-min, 6 frequent, 7 buffer,\n"
+        "         8 seeded (seeds=PATH)\n");
 }
 
 static bool parse_ratio(const std::string& s, int32_t& a, int32_t& b) {
@@ -124,6 +127,10 @@ int main(int argc, char** argv) {
                 throw glp::Error(GLP_E_ARG, "gamma must be num/den");
             if (method == GLP_SLPA) p.slpa_T = std::max(steps, 1);
             prop.reset(new glp::Propagator(&graph, p));
+            if (method == GLP_SEEDED) {
+                if (!kv.count("seeds")) throw glp::Error(GLP_E_ARG, "method 8 needs seeds=PATH");
+                prop->set_seeds(glp::read_seed_file(kv["seeds"], graph.n));
+            }
             if (replay) {
                 prop->run_steps(0);
                 glp::check(glp_session_step_graph(prop->session(), steps), "replay");

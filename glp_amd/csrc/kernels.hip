@@ -160,6 +160,12 @@ template <class H>
 __device__ __forceinline__ bool is_active(const StepArgs& a, int32_t v) {
     return !frontier_on<H>(a) || a.act_r[v] != 0;
 }
+// Seeded LP: a seed keeps its own label and is never recomputed.
+template <class H>
+__device__ __forceinline__ bool is_fixed(const StepArgs& a, int32_t v) {
+    if constexpr (seeded_of<H>::value) return a.seeds[v] != 0;
+    else return false;
+}
 template <class H>
 __device__ __forceinline__ bool marking(const StepArgs& a) {
     return H::kFrontier && a.act_w != nullptr && a.fctl[1] != 0;
@@ -269,7 +275,8 @@ __device__ __forceinline__ uint32_t packed_finish(const StepArgs& a, const Packe
         if (k < len && ok == s.key) cnt += of;
     }
     const int32_t vid = s.v0 + s.own;
-    uint64_t p = live ? H::label_score(a, vid, s.key, cnt) : 0ull;
+    // key 0 is an unlabeled neighbour (seeded LP): it never wins
+    uint64_t p = (live && s.key != 0u) ? H::label_score(a, vid, s.key, cnt) : 0ull;
     // segmented max: after the sweep the last lane of a segment holds its max
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -279,10 +286,12 @@ __device__ __forceinline__ uint32_t packed_finish(const StepArgs& a, const Packe
     const bool has = lane < s.vc && s.ve > s.vb;
     const uint64_t best = shfl_u64(p, has ? s.ve - 1 : lane);
     bool changed = false;
-    if (lane < s.vc) {
+    if (lane < s.vc && !is_fixed<H>(a, s.v0 + lane)) {
         const int32_t v = s.v0 + lane;
-        const int32_t nl = has ? packed_label(best) : s.old;
-        H::update_vertex(a, v, nl, has ? best : 0ull);
+        // no edge, or no counted label: the vertex keeps its label
+        const bool got = has && best != 0ull;
+        const int32_t nl = got ? packed_label(best) : s.old;
+        H::update_vertex(a, v, nl, got ? best : 0ull);
         changed = nl != s.old;
     }
     if (marking<H>(a)) {
@@ -393,7 +402,7 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int32_t* __re
     }
 #pragma unroll
     for (int p = 0; p < kHalfPairs; ++p) {
-        run[p] = run[p] && is_active<H>(a, v[p]);
+        run[p] = run[p] && is_active<H>(a, v[p]) && !is_fixed<H>(a, v[p]);
         b[p] = a.off[v[p]];
         e[p] = a.off[v[p] + 1];
         old[p] = a.keep[v[p]];
@@ -431,7 +440,7 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int32_t* __re
         wave_lds_sync();  // the next pair clears what this scan read
         bool changed = false;
         if (hl == 0 && run[p]) {
-            const int32_t nl = packed_label(best);
+            const int32_t nl = best ? packed_label(best) : old[p];
             H::update_vertex(a, v[p], nl, best);
             changed = nl != old[p];
             if (changed) note_changed(a, v[p]);
@@ -464,7 +473,7 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int32_t* __re
     for (int k = 0; k < kWaveNV; ++k) v[k] = verts[min(i0 + k, nv - 1)];
 #pragma unroll
     for (int k = 0; k < kWaveNV; ++k) {
-        run[k] = i0 + k < nv && is_active<H>(a, v[k]);
+        run[k] = i0 + k < nv && is_active<H>(a, v[k]) && !is_fixed<H>(a, v[k]);
         b[k] = a.off[v[k]];
         e[k] = a.off[v[k] + 1];
         old[k] = a.keep[v[k]];
@@ -504,7 +513,7 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int32_t* __re
         wave_lds_sync();  // the next vertex clears what this scan read
         bool changed = false;
         if (lane == 0) {
-            const int32_t nl = packed_label(best);
+            const int32_t nl = best ? packed_label(best) : old[k];
             H::update_vertex(a, v[k], nl, best);
             changed = nl != old[k];
             if (changed) note_changed(a, v[k]);
@@ -538,7 +547,7 @@ __device__ __forceinline__ void block_body(const StepArgs& a, const int32_t* __r
     uint32_t* K = s_dyn;
     uint32_t* C = s_dyn + cap_max;
     const int32_t v = verts[blk];
-    if (!is_active<H>(a, v)) return;  // block-uniform
+    if (!is_active<H>(a, v) || is_fixed<H>(a, v)) return;  // block-uniform
     const int32_t b = a.off[v], e = a.off[v + 1];
     const uint32_t cap = min(max(next_pow2(2u * (uint32_t)(e - b)), 64u), cap_max);
     const uint32_t mask = cap - 1u;
@@ -559,7 +568,7 @@ __device__ __forceinline__ void block_body(const StepArgs& a, const int32_t* __r
         bool changed = false;
         if (threadIdx.x == 0) {
             const int32_t old = a.keep[v];
-            const int32_t nl = packed_label(best);
+            const int32_t nl = best ? packed_label(best) : old;
             H::update_vertex(a, v, nl, best);
             changed = nl != old;
             if (changed) note_changed(a, v);
@@ -624,7 +633,7 @@ __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h
     const int32_t nch = (ve - vb + h.chunk - 1) / h.chunk;
     const uint32_t P = hub_buckets(ve - vb, h.part_keys);
     uint32_t* hist = h.hs.hist + h.hist_base[hc.x];  // [bucket][chunk]
-    if (!is_active<H>(a, v)) {  // block-uniform: the hub keeps its label
+    if (!is_active<H>(a, v) || is_fixed<H>(a, v)) {  // block-uniform: the hub keeps its label
         for (uint32_t q = threadIdx.x; q < P; q += kHubThreads) hist[(int64_t)q * nch + hc.y] = 0u;
         if (threadIdx.x == 0) h.hs.chunk_cnt[blk] = 0u;
         return;
@@ -879,9 +888,9 @@ __device__ __forceinline__ void hub_final_body(const StepArgs& a, const HubArgs&
         const int32_t v = h.hubs[i];
         const unsigned long long best = h.hs.best[i];
         h.hs.best[i] = 0ull;
-        if (is_active<H>(a, v)) {
+        if (is_active<H>(a, v) && !is_fixed<H>(a, v)) {
             const int32_t old = a.keep[v];
-            const int32_t nl = packed_label(best);
+            const int32_t nl = best ? packed_label(best) : old;
             H::update_vertex(a, v, nl, best);
             changed = nl != old;
             if (changed) note_changed(a, v);
@@ -1145,6 +1154,12 @@ __global__ void k_fill_i32(int32_t* p, int32_t value, int64_t count) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
         p[i] = value;
 }
+// Seeded LP's L_0 (header/kernel.cuh:453-466): v for a seed, -1 otherwise.
+__global__ void k_seed_labels(int32_t* p, const uint8_t* __restrict__ seeds, int64_t count) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
+        p[i] = seeds[i] ? (int32_t)i : -1;
+}
 __global__ void k_iota(int32_t* p, int64_t count) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
@@ -1306,6 +1321,7 @@ void init_kernels() {
     GLP_SETLDS4(LayeredHooks)
     GLP_SETLDS4(SlpaHooks)
     GLP_SETLDS4(UserHooks)
+    GLP_SETLDS4(SeededHooks)
 #undef GLP_SETLDS4
 #undef GLP_SETLDS_HUB
 #undef GLP_SETLDS
@@ -1332,6 +1348,7 @@ int launch_step(Method method, bool weighted, bool gathered, const DeviceGraph& 
         case Method::Layered: return launch_hooks<LayeredHooks>(weighted, gathered, g, hs, a, s, ev);
         case Method::Slpa: return launch_hooks<SlpaHooks>(weighted, gathered, g, hs, a, s, ev);
         case Method::User: return launch_hooks<UserHooks>(weighted, gathered, g, hs, a, s, ev);
+        case Method::Seeded: return launch_hooks<SeededHooks>(weighted, gathered, g, hs, a, s, ev);
     }
     return set_error(GLP_E_ARG, "unknown method");
 }
@@ -1362,6 +1379,14 @@ int launch_gather(const StepArgs& a, int64_t m, int32_t* adj, hipStream_t s) {
         hipLaunchKernelGGL(k_gather, dim3(grid), dim3(256), 0, s, a.nbr, a.L, adj, m);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "gather launch: %s", hipGetErrorString(e));
+}
+
+int launch_seed_labels(int32_t* p, const uint8_t* seeds, int64_t count, hipStream_t s) {
+    if (count <= 0) return GLP_OK;
+    const unsigned grid = (unsigned)std::min<int64_t>(grid_for(count, 256), 256 * 32);
+    hipLaunchKernelGGL(k_seed_labels, dim3(grid), dim3(256), 0, s, p, seeds, count);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "seed label launch: %s", hipGetErrorString(e));
 }
 
 int launch_fill_i32(int32_t* p, int32_t value, int64_t count, hipStream_t s) {

@@ -26,6 +26,8 @@ struct glp_session {
     int32_t* fctl = nullptr;             // frontier control {all_active, mark, threshold}
     int64_t act_bytes = 0;
     uint32_t* chg[2] = {nullptr, nullptr};  // frontier gather: changed-vertex bitmaps
+    uint8_t* seeds = nullptr;              // seeded LP: n bytes, nonzero = seed
+    bool seeds_set = false;
     int64_t chg_bytes = 0;
     glp::HubScratch hs;
     unsigned long long* d_acc = nullptr;   // change counter of the running step

@@ -126,15 +126,21 @@ static Method method_of(int32_t m) {
         case GLP_LAYERED: return Method::Layered;
         case GLP_SLPA: return Method::Slpa;
         case GLP_USER: return Method::User;
+        case GLP_SEEDED: return Method::Seeded;
         default: return Method::Classic;
     }
+}
+
+// Methods run by the sketch / table counting kernels (counting.hip).
+static bool counting_method(int32_t m) {
+    return m == GLP_COUNTMIN || m == GLP_FREQUENT || m == GLP_BUFFER;
 }
 
 int check_params(const HostCSR& g, const glp_params* p) {
     if (!p) return set_error(GLP_E_ARG, "null params");
     for (int i = 0; i < 5; ++i)
         if (p->reserved[i]) return set_error(GLP_E_ARG, "reserved params fields must be zero");
-    if (p->method < GLP_CLASSIC || p->method > GLP_BUFFER)
+    if (p->method < GLP_CLASSIC || p->method > GLP_SEEDED)
         return set_error(GLP_E_ARG, "unknown method %d", p->method);
     if (p->frontier && p->method != GLP_CLASSIC)
         return set_error(GLP_E_UNSUPPORTED,
@@ -168,8 +174,10 @@ int check_params(const HostCSR& g, const glp_params* p) {
     if (p->method == GLP_FREQUENT || p->method == GLP_BUFFER) {
         if (p->table_cap < 1 || p->table_cap > 64) return set_error(GLP_E_ARG, "table_cap must be in [1,64]");
     }
-    if (p->gather && p->method != GLP_CLASSIC && p->method != GLP_LAYERED && p->method != GLP_USER)
-        return set_error(GLP_E_UNSUPPORTED, "separate gather is implemented for classic, layered and user LP");
+    if (p->gather && p->method != GLP_CLASSIC && p->method != GLP_LAYERED && p->method != GLP_USER &&
+        p->method != GLP_SEEDED)
+        return set_error(GLP_E_UNSUPPORTED,
+                         "separate gather is implemented for classic, layered, user and seeded LP");
     return GLP_OK;
 }
 
@@ -198,6 +206,7 @@ glp_session::~glp_session() {
     (void)hipFree(act[1]);
     (void)hipFree(chg[0]);
     (void)hipFree(chg[1]);
+    (void)hipFree(seeds);
     (void)hipFree(fctl);
     (void)hipFree(hs.best);
     (void)hipFree(d_acc);
@@ -309,9 +318,14 @@ int session_reset(glp_session* s) {
     }
     s->host_changes.clear();
     int rc;
-    // L_0[v] = v in both buffers (RS: both label buffers start equal)
-    if ((rc = launch_iota(s->L[0], n, s->stream))) return rc;
-    if ((rc = launch_iota(s->L[1], n, s->stream))) return rc;
+    // L_0 in both buffers (RS: both label buffers start equal): v, or for
+    // seeded LP v on seeds and -1 elsewhere (all -1 until seeds are set)
+    for (int b = 0; b < 2; ++b) {
+        if (s->method != Method::Seeded) rc = launch_iota(s->L[b], n, s->stream);
+        else if (s->seeds_set) rc = launch_seed_labels(s->L[b], s->seeds, n, s->stream);
+        else rc = launch_fill_i32(s->L[b], -1, n, s->stream);
+        if (rc) return rc;
+    }
     if (s->V[0]) {
         // V_0(l) = #{u : L_0[u] = l} = 1
         if ((rc = launch_fill_i32(s->V[0], 1, n, s->stream))) return rc;
@@ -363,6 +377,12 @@ static StepArgs step_args(glp_session* s) {
             a.score_mul = s->p.gamma_den + s->p.gamma_num;
             a.score_vmul = s->p.gamma_num;
             break;
+        case Method::Seeded:
+            a.L = s->L[c];
+            a.keep = s->L[c];
+            a.Lw = s->L[c ^ 1];
+            a.seeds = s->seeds;
+            break;
         case Method::User:
             a.L = s->shown;
             a.keep = s->L[c];
@@ -390,7 +410,9 @@ static StepArgs step_args(glp_session* s) {
 
 // Enqueues one step.  ev: per-tier events or null.
 int enqueue_step(glp_session* s, hipEvent_t* ev) {
-    if (s->p.method >= GLP_COUNTMIN) return enqueue_counting_step(s);
+    if (counting_method(s->p.method)) return enqueue_counting_step(s);
+    if (s->method == Method::Seeded && !s->seeds_set)
+        return set_error(GLP_E_STATE, "a seeded session needs its seeds (glp_session_set_seeds)");
     if (s->method == Method::Slpa && s->t >= s->p.slpa_T)
         return set_error(GLP_E_STATE, "SLPA memory holds %d steps", s->p.slpa_T);
     StepArgs a = step_args(s);
@@ -475,7 +497,7 @@ int session_sync(glp_session* s) {
 }
 
 int session_step_graph(glp_session* s, int32_t steps) {
-    if (s->method == Method::Slpa || s->p.method >= GLP_COUNTMIN)
+    if (s->method == Method::Slpa || counting_method(s->p.method))
         return session_step(s, steps);  // step-dependent pointers: plain launches
     if (steps <= 0) return steps == 0 ? GLP_OK : set_error(GLP_E_ARG, "negative step count");
     // one captured graph per parity of the label buffers
@@ -719,9 +741,8 @@ extern "C" int glp_session_free(glp_session* s) {
     return GLP_OK;
 }
 
-extern "C" int glp_run(glp_graph* g, const glp_params* p, int32_t steps, int32_t* labels_out,
-                       int32_t* step_dump, glp_timing* timing) {
-    clear_error();
+static int run_impl(glp_graph* g, const glp_params* p, const uint8_t* seeds, int32_t steps,
+                    int32_t* labels_out, int32_t* step_dump, glp_timing* timing) {
     if (!g || !p || steps < 0 || (!labels_out && g->csr.n)) return set_error(GLP_E_ARG, "bad run arguments");
     glp_params q = *p;
     if (q.method == GLP_SLPA && q.slpa_T < steps) q.slpa_T = steps;
@@ -729,6 +750,7 @@ extern "C" int glp_run(glp_graph* g, const glp_params* p, int32_t steps, int32_t
     int rc = session_create(g, &q, &s);
     if (rc != GLP_OK) return rc;
     std::unique_ptr<glp_session> guard(s);
+    if (q.method == GLP_SEEDED && (rc = glp_session_set_seeds(s, seeds, s->n)) != GLP_OK) return rc;
     for (int32_t i = 0; i < steps; ++i) {
         if ((rc = session_step(s, 1)) != GLP_OK) return rc;
         if (step_dump) {
@@ -743,6 +765,31 @@ extern "C" int glp_run(glp_graph* g, const glp_params* p, int32_t steps, int32_t
     }
     if (timing) *timing = s->timing;
     return GLP_OK;
+}
+
+extern "C" int glp_run(glp_graph* g, const glp_params* p, int32_t steps, int32_t* labels_out,
+                       int32_t* step_dump, glp_timing* timing) {
+    clear_error();
+    if (p && p->method == GLP_SEEDED) return set_error(GLP_E_ARG, "seeded LP runs through glp_run_seeded");
+    return run_impl(g, p, nullptr, steps, labels_out, step_dump, timing);
+}
+
+extern "C" int glp_session_set_seeds(glp_session* s, const uint8_t* seeds, int32_t n) {
+    clear_error();
+    if (!s || (!seeds && n > 0)) return set_error(GLP_E_ARG, "bad seed arguments");
+    if (s->method != Method::Seeded) return set_error(GLP_E_STATE, "seeds exist for seeded sessions only");
+    if (n != s->n) return set_error(GLP_E_ARG, "seeds hold %d entries, the graph has %d vertices", n, s->n);
+    if (!s->seeds) GLP_HIP(hipMalloc(&s->seeds, std::max<size_t>((size_t)n, 1)));
+    if (n > 0) GLP_HIP(hipMemcpyAsync(s->seeds, seeds, (size_t)n, hipMemcpyHostToDevice, s->stream));
+    s->seeds_set = true;
+    return session_reset(s);  // synchronises the stream (the host buffer may go)
+}
+
+extern "C" int glp_run_seeded(glp_graph* g, const glp_params* p, const uint8_t* seeds, int32_t steps,
+                              int32_t* labels_out, int32_t* step_dump, glp_timing* timing) {
+    clear_error();
+    if (!g || !p || p->method != GLP_SEEDED) return set_error(GLP_E_ARG, "glp_run_seeded needs method GLP_SEEDED");
+    return run_impl(g, p, seeds, steps, labels_out, step_dump, timing);
 }
 
 extern "C" void glp_graph_free(glp_graph* g) {

@@ -42,7 +42,8 @@ enum {
     GLP_USER = 4,      /* prop/UserdefinedLprop.cuh: the compiled-in user hooks    */
     GLP_COUNTMIN = 5,  /* method/countmin.cuh SharedCM sketch counting             */
     GLP_FREQUENT = 6,  /* method/frequent.cuh SharedFA heavy-hitter table          */
-    GLP_BUFFER = 7     /* method/buffer.cuh Buffer decrement-by-x table            */
+    GLP_BUFFER = 7,    /* method/buffer.cuh Buffer decrement-by-x table            */
+    GLP_SEEDED = 8     /* prop/layeredLprop.cuh seed-constrained LP (seeds fixed)  */
 };
 
 /* ---- opaque handles --------------------------------------------------- */
@@ -53,7 +54,7 @@ typedef struct glp_server glp_server;     /* continuous-batching request loop   
 
 /* ---- run parameters --------------------------------------------------- */
 typedef struct glp_params {
-    int32_t method;       /* GLP_CLASSIC ... GLP_BUFFER                          */
+    int32_t method;       /* GLP_CLASSIC ... GLP_SEEDED                          */
     int32_t weighted;     /* 1: frequency = sum of uint32 edge weights           */
     int32_t gamma_num;    /* layered: gamma = gamma_num / gamma_den,             */
     int32_t gamma_den;    /*   gamma_den a power of two (dyadic, exact scores)   */
@@ -148,6 +149,10 @@ void glp_graph_free(glp_graph* g);
  * timing: may be NULL.                                                       */
 int glp_run(glp_graph* g, const glp_params* p, int32_t steps,
             int32_t* labels_out, int32_t* step_dump, glp_timing* timing);
+/* glp_run for seeded LP (p->method == GLP_SEEDED): seeds as in
+ * glp_session_set_seeds (n bytes, nonzero = seed).                          */
+int glp_run_seeded(glp_graph* g, const glp_params* p, const uint8_t* seeds, int32_t steps,
+                   int32_t* labels_out, int32_t* step_dump, glp_timing* timing);
 
 /* ---- sessions: device-resident state for repeated steps --------------- */
 /* Uploads the CSR (once per graph), plans tiers and allocates label state;
@@ -165,6 +170,13 @@ int glp_session_step_graph(glp_session* s, int32_t steps);
  * over (the reference's per-round delta schedule, prop/MultiLprop.cuh:308-314,
  * with gamma = 1/delta).                                                     */
 int glp_session_set_gamma(glp_session* s, int32_t gamma_num, int32_t gamma_den);
+/* Seeded LP (method GLP_SEEDED): replaces layeredLprop::assign_seeds_for_device
+ * and initialize_labels_by_seeds (prop/layeredLprop.cuh:161-180,222-224;
+ * header/kernel.cuh:453-466).  seeds: n bytes, nonzero marks a seed.  Copies
+ * them to the device and resets the session: L_0[v] = v for a seed, -1
+ * otherwise.  A seeded session must have its seeds set before its first
+ * step.                                                                      */
+int glp_session_set_seeds(glp_session* s, const uint8_t* seeds, int32_t n);
 /* Waits for the session's stream.                                           */
 int glp_session_sync(glp_session* s);
 /* Copies the current labels (n int32) to host; blocks.                      */

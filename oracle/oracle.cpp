@@ -12,6 +12,10 @@
 //                        brute force over all labelings of small graphs,
 //                        multigraph multiplicity and weighted cases
 //   orc_layered_step     gamma=0 reduces to classic; hand-derived scores
+//   orc_seeded_step      closed forms (paths, star, competing seeds, isolated
+//                        and unreachable vertices), all-seeds identity,
+//                        invariants (seeds fixed, labels are seed ids, the
+//                        labeled set only grows and advances one hop a step)
 //   orc_slpa / _post     T=1 reduces to one classic step; stream KATs
 //   orc_countmin_step    no-collision widths reduce to classic; aliasing
 //                        closed forms; hash KATs from the constants
@@ -136,6 +140,26 @@ extern "C" void orc_classic_step(int32_t n, const int32_t* off, const int32_t* n
         }
         std::map<int32_t, uint64_t> count;
         for (int32_t j = off[v]; j < off[v + 1]; ++j) count[prev[nbr[j]]] += w ? w[j] : 1u;
+        Best<uint64_t> b;
+        for (auto& kv : count) b.offer(kv.second, kv.first);
+        next[v] = b.l;
+    }
+}
+
+extern "C" void orc_seeded_step(int32_t n, const int32_t* off, const int32_t* nbr, const uint32_t* w,
+                                const uint8_t* seeds, const int32_t* prev, int32_t* next) {
+    for (int32_t v = 0; v < n; ++v) {
+        if (seeds[v]) {
+            next[v] = v;
+            continue;
+        }
+        std::map<int32_t, uint64_t> count;
+        for (int32_t j = off[v]; j < off[v + 1]; ++j)
+            if (prev[nbr[j]] >= 0) count[prev[nbr[j]]] += w ? w[j] : 1u;
+        if (count.empty()) {
+            next[v] = prev[v];
+            continue;
+        }
         Best<uint64_t> b;
         for (auto& kv : count) b.offer(kv.second, kv.first);
         next[v] = b.l;

@@ -39,6 +39,14 @@ void orc_layered_step(int32_t n, const int32_t* off, const int32_t* nbr, const u
                       int32_t p, int32_t q, const int32_t* prev, const int32_t* vprev,
                       int32_t* next, int32_t* vnext);
 
+/* RS seeded step (prop/layeredLprop.cuh seed-constrained LP; seeds fixed as
+ * in header/kernel.cuh:4490, unlabeled neighbours never counted as in
+ * :4468,3272,1784): a seed keeps its own id; another vertex
+ * takes the highest (frequency, label) over neighbours whose label is >= 0
+ * and keeps its label when it has none.  L_0 = v on seeds, -1 elsewhere. */
+void orc_seeded_step(int32_t n, const int32_t* off, const int32_t* nbr, const uint32_t* w,
+                     const uint8_t* seeds, const int32_t* prev, int32_t* next);
+
 /* SLPA speaker stream R(seed,t,u). */
 uint32_t orc_stream_u32(uint64_t seed, int32_t t, int32_t u);
 /* SLPA (prop/SLPAprop.cuh:337-398 structure, speaker/listener rule defined

@@ -27,6 +27,7 @@ def _s(name, res, *args):
 _s("orc_load_bin_size", ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _i32p, _i64p)
 _s("orc_load_bin", ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _u32p)
 _s("orc_classic_step", None, ctypes.c_int32, _i32p, _i32p, _u32p, _i32p, _i32p)
+_s("orc_seeded_step", None, ctypes.c_int32, _i32p, _i32p, _u32p, ctypes.c_void_p, _i32p, _i32p)
 _s("orc_layered_step", None, ctypes.c_int32, _i32p, _i32p, _u32p, ctypes.c_int32, ctypes.c_int32,
    _i32p, _i32p, _i32p, _i32p)
 _s("orc_stream_u32", ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32)
@@ -87,6 +88,27 @@ def classic(off, nbr, steps, w=None, labels=None):
     out = []
     for _ in range(steps):
         L = classic_step(off, nbr, L, w)
+        out.append(L)
+    return np.array(out).reshape(steps, n)
+
+
+def seeded_labels0(seeds):
+    """Seeded LP's L_0: v on seeds, -1 elsewhere."""
+    seeds = np.asarray(seeds, dtype=np.uint8)
+    return np.where(seeds != 0, np.arange(len(seeds), dtype=np.int32), -1).astype(np.int32)
+
+
+def seeded(off, nbr, steps, seeds, w=None):
+    """Per-step labels of seeded LP, shape (steps, n)."""
+    off, nbr, w = _csr(off, nbr, w)
+    n = len(off) - 1
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint8)
+    L = seeded_labels0(seeds)
+    out = []
+    for _ in range(steps):
+        nxt = np.zeros(max(n, 1), dtype=np.int32)
+        _o.orc_seeded_step(n, _i(off), _i(nbr), _u(w), seeds.ctypes.data, _i(L), _i(nxt))
+        L = nxt[:n]
         out.append(L)
     return np.array(out).reshape(steps, n)
 

@@ -90,3 +90,23 @@ def test_autotune_keeps_results(tmp_path):
     finally:
         for k, v in saved.items():
             G.tuning_set(k, v)
+
+
+def test_seeded_method_with_seed_file(graph_file, tmp_path):
+    """Method 8 reads a seed file (one vertex id per line, as
+    layeredLprop::assign_seeds_for_device does) and runs niter-1 steps."""
+    off, nbr, _ = O.load_bin(graph_file)
+    n = len(off) - 1
+    seeds = np.zeros(n, dtype=np.uint8)
+    ids = np.random.default_rng(7).choice(n, size=40, replace=False)
+    seeds[ids] = 1
+    sf = tmp_path / "seeds.txt"
+    sf.write_text("".join("%d\n" % i for i in ids))
+    out = tmp_path / "s.log"
+    lab = tmp_path / "s.bin"
+    gpulp(8, 5, graph_file, out, 0, "seeds=%s" % sf, "labels=%s" % lab)
+    assert np.array_equal(np.fromfile(lab, dtype=np.int32), O.seeded(off, nbr, 4, seeds)[-1])
+    assert "method: 8 iteration: 5 time: " in out.read_text()
+    r = subprocess.run([os.path.join(BIN, "GPULP"), "8", "3", graph_file, str(out), "0"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "seeds" in r.stderr

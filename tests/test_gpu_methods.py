@@ -142,6 +142,64 @@ def test_user_hooks_are_classic(rmat14):
     assert np.array_equal(dump, dump2)
 
 
+# ------------------------------------------------------------------- seeded ----
+
+@pytest.mark.parametrize("frac,gather", [(0.05, False), (0.002, False), (0.05, True)])
+def test_seeded_matches_oracle(rmat14, frac, gather):
+    g, (off, nbr, _) = rmat14
+    seeds = (np.random.default_rng(41).random(len(off) - 1) < frac).astype(np.uint8)
+    _, dump, _ = G.run(g, 8, dump=True, method=G.SEEDED, seeds=seeds, gather=gather)
+    assert_steps_equal(dump, O.seeded(off, nbr, 8, seeds), "seeded")
+
+
+def test_seeded_weighted_and_every_tier(rmat13w):
+    g, (off, nbr, w) = rmat13w
+    seeds = (np.random.default_rng(42).random(len(off) - 1) < 0.03).astype(np.uint8)
+    _, dump, _ = G.run(g, 6, dump=True, method=G.SEEDED, seeds=seeds, weighted=True)
+    assert_steps_equal(dump, O.seeded(off, nbr, 6, seeds, w=w), "seeded weighted")
+    # tiny thresholds: most vertices (and some seeds) in the block and hub tiers
+    with G.tuning(pack_max_deg=4, half_max_deg=8, wave_max_deg=16, block_max_deg=32, hub_chunk=256,
+                  part_keys=256):
+        gt = G.Graph.rmat(14, 16, seed=43)
+        offt, nbrt, _ = gt.csr()
+        deg = np.diff(offt)
+        seeds = (np.random.default_rng(44).random(len(deg)) < 0.02).astype(np.uint8)
+        seeds[np.argsort(-deg)[:3]] = 1                       # hubs among the seeds
+        _, dump, t = G.run(gt, 6, dump=True, method=G.SEEDED, seeds=seeds)
+    assert t["tier_vertices"]["hub"] > 0
+    assert_steps_equal(dump, O.seeded(offt, nbrt, 6, seeds), "seeded tiers")
+
+
+def test_seeded_session_replay_checkpoint_and_errors(rmat14):
+    g, (off, nbr, _) = rmat14
+    n = len(off) - 1
+    seeds = (np.random.default_rng(45).random(n) < 0.01).astype(np.uint8)
+    want = O.seeded(off, nbr, 9, seeds)
+    with G.Session(g, method=G.SEEDED) as s:
+        with pytest.raises(G.GLPError):
+            s.step(1)                                        # no seeds yet
+        s.set_seeds(seeds)
+        assert np.array_equal(s.labels(), O.seeded_labels0(seeds))
+        s.step(2)
+        s.step(3, graph=True)
+        assert np.array_equal(s.labels(), want[4])
+        state = s.save()
+        with pytest.raises(G.GLPError):
+            s.set_seeds(seeds[:-1])                          # wrong length
+    with G.Session(g, method=G.SEEDED) as s2:
+        s2.set_seeds(seeds)
+        s2.load(state)
+        s2.step(4)
+        assert np.array_equal(s2.labels(), want[8])
+    with G.Session(g) as s3:
+        with pytest.raises(G.GLPError):
+            s3.set_seeds(seeds)                              # not a seeded session
+    with pytest.raises(G.GLPError):
+        G.run(g, 2, method=G.SEEDED, seeds=seeds, frontier=True)
+    with pytest.raises(G.GLPError):
+        G.Batch(mix_graphs(2, 1), method=G.SEEDED)
+
+
 # ------------------------------------------------------------------ batched ----
 
 def mix_graphs(count, seed, nmax=6000):

@@ -243,3 +243,81 @@ def test_tables_hand_trace():
     # unweighted MG, 2 slots: third label cancels both; table empty -> keep
     assert O.table_step(off, nbr, prev, 2, 0)[0] == 0
     assert O.table_step(off, nbr, prev, 3, 1, w)[0] == 12
+
+
+# ------------------------------------------------------------------ seeded ----
+
+def test_seeded_path_one_hop_per_step():
+    off, nbr, _ = O.csr_from_edges(5, [(0, 1), (1, 2), (2, 3), (3, 4)])
+    got = O.seeded(off, nbr, 4, [1, 0, 0, 0, 0])
+    assert got.tolist() == [[0, 0, -1, -1, -1], [0, 0, 0, -1, -1], [0, 0, 0, 0, -1], [0, 0, 0, 0, 0]]
+
+
+def test_seeded_competing_seeds_tie_to_higher_label():
+    """Seeds 0 and 6 at the ends of a 7-path meet in the middle: vertex 3 sees
+    one neighbour with each label and takes the higher (6)."""
+    off, nbr, _ = O.csr_from_edges(7, [(i, i + 1) for i in range(6)])
+    got = O.seeded(off, nbr, 4, [1, 0, 0, 0, 0, 0, 1])
+    assert got[0].tolist() == [0, 0, -1, -1, -1, 6, 6]
+    assert got[1].tolist() == [0, 0, 0, -1, 6, 6, 6]
+    assert got[2].tolist() == [0, 0, 0, 6, 6, 6, 6]
+    # vertex 2 then sees 0 (from 1) and 6 (from 3): tie, 6; vertex 4 sees 6 twice
+    assert got[3].tolist() == [0, 0, 6, 6, 6, 6, 6]
+
+
+def test_seeded_star_and_isolated():
+    """Leaves 1..5 seeded: the centre takes the highest leaf; an isolated
+    non-seed stays unlabeled, an isolated seed keeps its id; seeds never move."""
+    off, nbr, _ = O.csr_from_edges(8, [(0, i) for i in range(1, 6)])
+    seeds = [0, 1, 1, 1, 1, 1, 0, 1]
+    got = O.seeded(off, nbr, 3, seeds)
+    for t in range(3):
+        assert got[t].tolist() == [5, 1, 2, 3, 4, 5, -1, 7]
+
+
+def test_seeded_weights_decide():
+    off, nbr, w = O.csr_from_edges(3, [(0, 1), (1, 2)], weights=[3, 1])
+    got = O.seeded(off, nbr, 1, [1, 0, 1], w=w)
+    assert got[0].tolist() == [0, 0, 2]     # 3 > 1 outweighs the higher label
+    off, nbr, w = O.csr_from_edges(3, [(0, 1), (1, 2)], weights=[1, 1])
+    assert O.seeded(off, nbr, 1, [1, 0, 1], w=w)[0].tolist() == [0, 2, 2]
+
+
+def test_seeded_extremes():
+    rng = np.random.default_rng(4)
+    e = [tuple(x) for x in rng.integers(0, 200, size=(800, 2))]
+    off, nbr, _ = O.csr_from_edges(200, e)
+    n = 200
+    assert (O.seeded(off, nbr, 3, np.ones(n)) == np.arange(n)).all()
+    assert (O.seeded(off, nbr, 3, np.zeros(n)) == -1).all()
+
+
+def test_seeded_invariants_random():
+    """Seeds stay fixed, every label is -1 or a seed id, and the labeled set
+    after t steps is exactly the vertices within t hops of a seed (BFS)."""
+    rng = np.random.default_rng(5)
+    for trial in range(5):
+        n = 400
+        e = [tuple(x) for x in rng.integers(0, n, size=(700, 2))]
+        off, nbr, _ = O.csr_from_edges(n, e)
+        seeds = (rng.random(n) < 0.02).astype(np.uint8)
+        got = O.seeded(off, nbr, 6, seeds)
+        dist = np.full(n, -1)
+        frontier = list(np.flatnonzero(seeds))
+        dist[frontier] = 0
+        d = 0
+        while frontier:
+            d += 1
+            nxt = []
+            for u in frontier:
+                for v in nbr[off[u]:off[u + 1]]:
+                    if dist[v] < 0:
+                        dist[v] = d
+                        nxt.append(v)
+            frontier = nxt
+        seed_ids = set(np.flatnonzero(seeds).tolist()) | {-1}
+        for t in range(6):
+            assert (got[t][seeds != 0] == np.flatnonzero(seeds)).all()
+            assert set(got[t].tolist()) <= seed_ids
+            labeled = got[t] >= 0
+            assert np.array_equal(labeled, (dist >= 0) & (dist <= t + 1))
