@@ -45,27 +45,42 @@ def tier_bytes(edges, verts, weighted=False):
 
 def cpu_baseline(off, nbr, target_s, threads):
     """The reference CPU loop (tutorial/CPUlp.cuh:28-92 sync_run_1, restated:
-    OpenMP over vertices, a std::map count per vertex, live label reads),
-    timed over a strided sample of the same graph's vertices from L0: every
-    k-th vertex, k chosen from a 1/256 pilot so the timed sample takes about
-    target_s.  The rate is the sample's edges over its time."""
+    OpenMP over vertices in id order with its default static schedule, a
+    std::map count per vertex, live label reads), timed over a strided sample
+    of the same graph's vertices from L0: every k-th vertex, k chosen from a
+    1/256 pilot so the timed sample takes about target_s.  With degree-sorted
+    ids the static schedule gives the first thread the hubs, in a sample as
+    in a whole step, so the sample's rate (its edges over its time) is the
+    whole step's.  The same sample with dynamic chunks (a balanced schedule
+    the reference does not use) is reported beside it."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     n = len(off) - 1
     deg = np.diff(off.astype(np.int64))
+    m = int(deg.sum())
+
+    def timed(verts, balanced):
+        lab = np.arange(n, dtype=np.int32)
+        return O.cpulp_sample(off, nbr, lab, verts, threads, balanced)
+
     pilot = np.arange(0, n, 256, dtype=np.int32)
-    lab = np.arange(n, dtype=np.int32)
-    secs_p = O.cpulp_sample(off, nbr, lab, pilot, threads)
-    rate = max(int(deg[pilot].sum()), 1) / max(secs_p, 1e-6)
-    stride = int(min(256, max(1, round(int(deg.sum()) / (rate * target_s)))))
+    rate = max(int(deg[pilot].sum()), 1) / max(timed(pilot, False), 1e-6)
+    stride = int(min(256, max(1, round(m / (rate * target_s)))))
     verts = np.arange(0, n, stride, dtype=np.int32)
-    lab = np.arange(n, dtype=np.int32)
-    secs = O.cpulp_sample(off, nbr, lab, verts, threads)
     e = int(deg[verts].sum())
+    secs = timed(verts, False)
+    # the balanced schedule on a sample of about a third of the time budget
+    rate_b = max(int(deg[pilot].sum()), 1) / max(timed(pilot, True), 1e-6)
+    stride_b = int(min(256, max(1, round(m / (rate_b * target_s / 3)))))
+    verts_b = np.arange(0, n, stride_b, dtype=np.int32)
+    secs_b = timed(verts_b, True)
     return {"value": e / secs / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
             "sample": "one verbatim CPUlp step from L0 over every %d-th vertex (%d vertices, %d "
-                      "edges of %d) of the same graph, %.2f s on %d threads"
-                      % (stride, len(verts), e, int(deg.sum()), secs, threads)}
+                      "edges of %d) of the same graph, static OpenMP schedule as the reference, "
+                      "%.2f s on %d threads" % (stride, len(verts), e, m, secs, threads),
+            "balanced_schedule_value": int(deg[verts_b].sum()) / secs_b / 1e9,
+            "balanced_schedule_sample": "every %d-th vertex, dynamic chunks of 64, %.2f s"
+                                        % (stride_b, secs_b)}
 
 
 def bench_batched(args, G):
@@ -211,6 +226,8 @@ def main():
     ap.add_argument("--graph-replay", action="store_true", help="replay steps from a HIP graph")
     ap.add_argument("--gather-steps", type=int, default=3,
                     help="steps of the separate-gather mode timed after the main run (0: skip)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the layered (same graph) and SLPA (LFR-style 2^20) legs")
     ap.add_argument("--frontier", type=int, default=1,
                     help="1: also time the exact-frontier mode on the same steps (reported apart)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
@@ -344,6 +361,42 @@ def main():
         log("frontier gather %.3f ms/step, labels identical: %s" % (gms_f, same_g))
         fg.free()
 
+    # the other methods of the same configurations: layered LP on this graph
+    # (gamma 0, 1/4, 1; same warmup and timed steps) and SLPA (T=20, r=0.1)
+    # on the LFR-style graph of 2^20 vertices
+    extra = None
+    if not args.no_extra:
+        extra = {"layered": {}}
+        for gn, gd in ((0, 1), (1, 4), (1, 1)):
+            ls = G.Session(g, G.make_params(method=G.LAYERED, gamma=(gn, gd)))
+            ls.step(args.warmup)
+            ls.sync()
+            t0 = time.perf_counter()
+            ls.step(args.steps)
+            ls.sync()
+            lms = (time.perf_counter() - t0) * 1e3 / args.steps
+            extra["layered"]["%d/%d" % (gn, gd)] = {"ms_per_step": round(lms, 4),
+                                                    "GTEPS": round(m / (lms / 1e3) / 1e9, 3)}
+            log("layered gamma=%d/%d: %.3f ms/step" % (gn, gd, lms))
+            ls.free()
+        lg = G.Graph.lfr(1 << 20, seed=args.seed)
+        ss = G.Session(lg, G.make_params(method=G.SLPA, slpa_T=20, slpa_r=(1, 10), seed=args.seed))
+        t0 = time.perf_counter()
+        ss.step(20)
+        ss.sync()
+        sms = (time.perf_counter() - t0) * 1e3 / 20
+        t1 = time.perf_counter()
+        dom, ns, _ = ss.slpa_post(8)
+        pms = (time.perf_counter() - t1) * 1e3
+        extra["slpa"] = {"graph": "LFR-style n=2^20 (tau1 2.5, degrees [8,512], tau2 1.5, sizes "
+                                  "[20,5000], mu 0.3), m=%d" % lg.m,
+                         "steps": 20, "ms_per_step": round(sms, 4),
+                         "GTEPS": round(lg.m / (sms / 1e3) / 1e9, 3), "post_ms": round(pms, 2),
+                         "mean_communities_per_vertex": round(float(ns.mean()), 3)}
+        log("slpa LFR-1M: %.3f ms/step, post %.1f ms" % (sms, pms))
+        ss.free()
+        lg.free()
+
     # the separate gather pass (LoadNeighbor as its own kernel): 12 B per edge
     gather = None
     if args.gather_steps > 0:
@@ -390,6 +443,7 @@ def main():
         "step_roofline": step_roofline,
         "gather": gather,
         "frontier": frontier,
+        "other_methods": extra,
         "tiers_ms": {k: round(v["ms"], 4) for k, v in tiers.items()},
         "changes_per_step": changes[-args.steps:],
     }

@@ -159,6 +159,9 @@ int launch_pick(Method method, const StepArgs& a, int32_t* shown, hipStream_t s)
 int launch_gather(const StepArgs& a, int64_t m, int32_t* adj, hipStream_t s);
 int launch_fill_i32(int32_t* p, int32_t value, int64_t count, hipStream_t s);
 int launch_iota(int32_t* p, int64_t count, hipStream_t s);
+// V[l] += #{v < n : L[v] = l} (layered LP's volume of the written labels;
+// V zeroed first).
+int launch_volume(const int32_t* L, int32_t* V, int32_t n, hipStream_t s);
 // p[i] = seeds[i] ? i : -1 (seeded LP's initial labels).
 int launch_seed_labels(int32_t* p, const uint8_t* seeds, int64_t count, hipStream_t s);
 int launch_end_step(unsigned long long* acc, unsigned long long* ring, int32_t* step_ctr,

@@ -106,7 +106,9 @@ struct ClassicHooks {
 // (prop/MultiLprop.cuh:313-314, header/kernel.cuh:2579 delta*count - gcounter)
 struct LayeredHooks {
     static constexpr bool kPick = false;
-    static constexpr bool kSkipIsolated = false;  // isolated vertices still add to V
+    // an isolated vertex keeps its label in both buffers; the volume pass
+    // counts it from the written buffer, so it needs no work
+    static constexpr bool kSkipIsolated = true;
     static constexpr bool kFrontier = false;      // V changes every label's score
     __device__ static __forceinline__ int32_t pick_label(const StepArgs& a, int32_t vid) {
         return a.keep[vid];
@@ -125,11 +127,12 @@ struct LayeredHooks {
         if (a.score_vmul) s -= a.score_vmul * a.Vr[key - 1];
         return pack_score(s, key);
     }
+    // the write-side volume V_t (header/kernel.cuh:2626 gcounter_temp) is the
+    // histogram of the labels written; k_volume computes it after the tiers
     __device__ static __forceinline__ void update_vertex(const StepArgs& a, int32_t vid,
                                                          int32_t label, uint64_t score) {
         (void)score;
         a.Lw[vid] = label;
-        if (a.Vw) atomicAdd(&a.Vw[label], 1);  // header/kernel.cuh:2626 gcounter_temp
     }
 };
 

@@ -187,14 +187,28 @@ __device__ __forceinline__ void mark_vertex(const StepArgs& a, int32_t v, int32_
 // every group's before any group is counted, so the dependent loads of the
 // groups (offsets, neighbour ids, labels) overlap; otherwise, and with the
 // frontier on, the groups go one by one through the guarded path.
+//
+// Counting a group: every edge lane inserts (owner slot, label) into a
+// 128-slot LDS table (<= 64 keys, so a slot is always free) and adds its
+// frequency; it reads back the sum for its pair, scores it, and an LDS
+// 64-bit max per owner slot keeps each vertex's best packed score.  Sums and
+// maxima are order-free integers, as in every tier.
 constexpr int kPackNG = 2;
+constexpr uint32_t kPackSlots = 128;
 
 // One group's state between its loads and its count.  Vertex lane i owns
 // edge lanes [vb, ve) of the group; edge lane `lane` belongs to vertex slot
-// `own`, whose edges are lanes [sb, se).
+// `own`.
 struct PackedGroup {
-    int32_t v0, vc, base, ve, vb, nE, own, sb, se, old;
+    int32_t v0, vc, base, ve, vb, nE, own, old;
     uint32_t key, fr;
+};
+
+// Per-wave LDS of the packed count.
+struct PackedLds {
+    unsigned long long key[kPackSlots];  // (owner slot + 1) << 32 | label key; 0 = free
+    uint32_t cnt[kPackSlots];
+    unsigned long long best[64];         // per vertex slot: best packed score
 };
 
 __device__ __forceinline__ void packed_segments(PackedGroup& s, int lane) {
@@ -211,8 +225,6 @@ __device__ __forceinline__ void packed_segments(PackedGroup& s, int lane) {
         else hi = mid;
     }
     s.own = lo;
-    s.sb = shfl_i32(s.vb, s.own);
-    s.se = shfl_i32(s.ve, s.own);
 }
 
 // Loads of a group with vc >= 1 vertices and nE >= 1 edges: indices past the
@@ -262,42 +274,56 @@ __device__ __forceinline__ bool packed_load_safe(const StepArgs& a, int2 gr, Pac
 // Counts, picks and writes the group's labels (all lanes active); returns
 // how many of them changed.
 template <class H, bool W, bool G>
-__device__ __forceinline__ uint32_t packed_finish(const StepArgs& a, const PackedGroup& s) {
+__device__ __forceinline__ uint32_t packed_finish(const StepArgs& a, const PackedGroup& s, PackedLds& t) {
     const int lane = lane_id();
-    const bool live = lane < s.nE;
-    // frequency of my label inside my vertex's segment
-    const int32_t len = live ? s.se - s.sb : 0;
-    uint32_t cnt = 0u;
-    for (int k = 0; __ballot(k < len) != 0ull; ++k) {
-        const int src = min(s.sb + k, 63);
-        const uint32_t ok = shfl_u32(s.key, src);
-        const uint32_t of = W ? shfl_u32(s.fr, src) : 1u;
-        if (k < len && ok == s.key) cnt += of;
+    // key 0 is an unlabeled neighbour (seeded LP): it never counts
+    const bool live = lane < s.nE && s.key != 0u;
+    wave_lds_sync();  // the previous group's reads of the table are done
+    t.key[lane] = 0ull;
+    t.key[lane + 64] = 0ull;
+    t.cnt[lane] = 0u;
+    t.cnt[lane + 64] = 0u;
+    t.best[lane] = 0ull;
+    wave_lds_sync();
+    uint32_t h = 0u;
+    bool claimed = false;  // this lane created the (owner, label) slot
+    if (live) {
+        const unsigned long long k64 = ((unsigned long long)(uint32_t)(s.own + 1) << 32) | s.key;
+        h = fmix32(s.key ^ ((uint32_t)s.own * 0x9e3779b9u)) & (kPackSlots - 1u);
+        for (;;) {
+            unsigned long long cur = lds_load(&t.key[h]);
+            if (cur == 0ull) {
+                cur = atomicCAS(&t.key[h], 0ull, k64);
+                if (cur == 0ull) {
+                    claimed = true;
+                    break;
+                }
+            }
+            if (cur == k64) break;
+            h = (h + 1u) & (kPackSlots - 1u);
+        }
+        atomicAdd(&t.cnt[h], W ? s.fr : 1u);
     }
-    const int32_t vid = s.v0 + s.own;
-    // key 0 is an unlabeled neighbour (seeded LP): it never wins
-    uint64_t p = (live && s.key != 0u) ? H::label_score(a, vid, s.key, cnt) : 0ull;
-    // segmented max: after the sweep the last lane of a segment holds its max
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint64_t o = shfl_u64(p, lane >= d ? lane - d : lane);
-        if (lane - d >= s.sb) p = max_u64(p, o);
+    wave_lds_sync();
+    // one score per distinct (owner, label): the slot's creator
+    if (claimed) {
+        const uint64_t p = H::label_score(a, s.v0 + s.own, s.key, t.cnt[h]);
+        atomicMax(&t.best[s.own], (unsigned long long)p);
     }
-    const bool has = lane < s.vc && s.ve > s.vb;
-    const uint64_t best = shfl_u64(p, has ? s.ve - 1 : lane);
+    wave_lds_sync();
     bool changed = false;
     if (lane < s.vc && !is_fixed<H>(a, s.v0 + lane)) {
         const int32_t v = s.v0 + lane;
         // no edge, or no counted label: the vertex keeps its label
-        const bool got = has && best != 0ull;
-        const int32_t nl = got ? packed_label(best) : s.old;
-        H::update_vertex(a, v, nl, got ? best : 0ull);
+        const uint64_t best = t.best[lane];
+        const int32_t nl = best ? packed_label(best) : s.old;
+        H::update_vertex(a, v, nl, best);
         changed = nl != s.old;
     }
     if (marking<H>(a)) {
         if (changed) a.act_w[s.v0 + lane] = 1;
         const bool owner_changed = shfl_i32(changed ? 1 : 0, s.own) != 0;
-        if (live && owner_changed) a.act_w[a.nbr[s.base + lane]] = 1;
+        if (lane < s.nE && owner_changed) a.act_w[a.nbr[s.base + lane]] = 1;
     }
     const unsigned long long bal = __ballot(changed);
     note_changed_run(a, s.v0, bal);
@@ -308,21 +334,23 @@ __device__ __forceinline__ uint32_t packed_finish(const StepArgs& a, const Packe
 template <class H, bool W, bool G>
 __device__ __forceinline__ void packed_body(const StepArgs& a, const int2* __restrict__ groups,
                                             int32_t ngroups, int32_t nge, uint32_t blk) {
+    __shared__ PackedLds s_pk[4];
     // wave-uniform first group (readfirstlane keeps the group loads scalar)
     const int32_t g0 =
         __builtin_amdgcn_readfirstlane((int32_t)((blk * 256u + threadIdx.x) >> 6)) * kPackNG;
     if (g0 >= ngroups) return;
+    PackedLds& t = s_pk[threadIdx.x >> 6];
     uint32_t nch = 0u;
     if (g0 + kPackNG <= nge && !frontier_on<H>(a)) {
         PackedGroup s[kPackNG];
 #pragma unroll
         for (int g = 0; g < kPackNG; ++g) packed_load_fast<H, W, G>(a, groups[g0 + g], s[g]);
 #pragma unroll
-        for (int g = 0; g < kPackNG; ++g) nch += packed_finish<H, W, G>(a, s[g]);
+        for (int g = 0; g < kPackNG; ++g) nch += packed_finish<H, W, G>(a, s[g], t);
     } else {
         for (int g = 0; g < kPackNG && g0 + g < ngroups; ++g) {
             PackedGroup s;
-            if (packed_load_safe<H, W, G>(a, groups[g0 + g], s)) nch += packed_finish<H, W, G>(a, s);
+            if (packed_load_safe<H, W, G>(a, groups[g0 + g], s)) nch += packed_finish<H, W, G>(a, s, t);
         }
     }
     add_changes(a, nch);
@@ -336,58 +364,41 @@ __device__ __forceinline__ void packed_body(const StepArgs& a, const int2* __res
 constexpr int kHalfPairs = 2;
 constexpr uint32_t kHalfCap = 2 * kHalfMaxDeg;
 
-// Reductions inside each 32-lane half (xor distances below 32).
-__device__ __forceinline__ uint64_t half_max_u64(uint64_t x) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int d = 16; d >= 1; d >>= 1) x = max_u64(x, shfl_u64(x, lane ^ d));
-    return x;
-}
-__device__ __forceinline__ uint32_t half_sum_u32(uint32_t x) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int d = 16; d >= 1; d >>= 1) x += shfl_u32(x, lane ^ d);
-    return x;
+// Inserts (key, f) into an LDS table of (key << 32 | frequency sum) slots
+// and returns the slot, with kClaimed set when this insert created it.  A
+// key claims a free slot with its first frequency already in it; later ones
+// add to the low word (sums stay below 2^31, so they never carry into the
+// key).  The table always has a free slot.
+constexpr uint32_t kClaimed = 0x80000000u;
+__device__ __forceinline__ uint32_t kc_insert(unsigned long long* T, uint32_t mask, uint32_t key,
+                                              uint32_t f) {
+    uint32_t h = fmix32(key) & mask;
+    const unsigned long long mine = (unsigned long long)key << 32;
+    for (;;) {
+        unsigned long long cur = lds_load(&T[h]);
+        if (cur == 0ull) {
+            cur = atomicCAS(&T[h], 0ull, mine | f);
+            if (cur == 0ull) return h | kClaimed;
+        }
+        if ((uint32_t)(cur >> 32) == key) {
+            atomicAdd(&T[h], (unsigned long long)f);
+            return h;
+        }
+        h = (h + 1u) & mask;
+    }
 }
 
-// wave_insert for two independent halves (Kt/Ct point at the lane's half's
-// table): each half merges the lanes holding its first pending lane's key.
-template <bool W>
-__device__ __forceinline__ void half_insert(uint32_t* Kt, uint32_t* Ct, uint32_t key, uint32_t f) {
-    const int lane = lane_id();
-    const bool upper = lane >= 32;
-    uint32_t ik = key, iff = f;  // what this lane finally inserts
-#pragma unroll
-    for (int r = 0; r < kPeels; ++r) {
-        const unsigned long long rest = __ballot(key != 0u);
-        if (rest == 0ull) break;
-        const uint32_t rlo = (uint32_t)rest, rhi = (uint32_t)(rest >> 32);
-        const int flo = rlo ? (int)__builtin_ctz(rlo) : 0;
-        const int fhi = rhi ? 32 + (int)__builtin_ctz(rhi) : 32;
-        const uint32_t llo = rlo ? (uint32_t)__builtin_amdgcn_readlane((int)key, flo) : 0u;
-        const uint32_t lhi = rhi ? (uint32_t)__builtin_amdgcn_readlane((int)key, fhi) : 0u;
-        const uint32_t lead = upper ? lhi : llo;
-        const bool same = key != 0u && key == lead;
-        const unsigned long long sm = __ballot(same);
-        const uint32_t clo = (uint32_t)__popc((uint32_t)sm), chi = (uint32_t)__popc((uint32_t)(sm >> 32));
-        const uint32_t cnt = upper ? chi : clo;
-        const uint32_t tot = W ? half_sum_u32(same ? f : 0u) : cnt;
-        if (same) {
-            if (cnt >= 2u) {  // a singleton keeps its own insert
-                ik = lane == (upper ? fhi : flo) ? lead : 0u;
-                iff = tot;
-            }
-            key = 0u;
-        }
-    }
-    if (ik) lds_insert(Kt, Ct, kHalfCap - 1u, ik, iff);
-}
+// Per-wave LDS of the half tier: one (key << 32 | sum) table per half and
+// each half's best packed score.
+struct HalfLds {
+    unsigned long long slot[2][kHalfCap];
+    unsigned long long best[2];
+};
 
 template <class H, bool W, bool G>
 __device__ __forceinline__ void half_body(const StepArgs& a, const int32_t* __restrict__ verts,
                                           int32_t nv, uint32_t blk) {
-    __shared__ uint32_t s_key[4][2][kHalfCap];
-    __shared__ uint32_t s_cnt[4][2][kHalfCap];
+    __shared__ HalfLds s_half[4];
     const int lane = lane_id(), wv = threadIdx.x >> 6, hf = lane >> 5, hl = lane & 31;
     const int32_t i0 = ((int32_t)blk * 4 + wv) * (2 * kHalfPairs);
     if (i0 >= nv) return;  // wave-uniform
@@ -417,27 +428,59 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int32_t* __re
             key[p][i] = H::template load_neighbor<W, G>(a, v[p], run[p] ? min(j, e[p] - 1) : b[p], &fr[p][i]);
             if (!run[p] || j >= e[p]) key[p][i] = 0u;
         }
-    uint32_t* K = s_key[wv][hf];
-    uint32_t* C = s_cnt[wv][hf];
+    HalfLds& t = s_half[wv];
+    unsigned long long* T = t.slot[hf];
     uint32_t nch = 0u;
 #pragma unroll
     for (int p = 0; p < kHalfPairs; ++p) {
         if (__ballot(run[p]) == 0ull) continue;  // wave-uniform: both halves idle
-        for (uint32_t i = hl; i < kHalfCap; i += 32) {
-            K[i] = 0u;
-            C[i] = 0u;
-        }
-        wave_lds_sync();
+        wave_lds_sync();  // the previous pair's reads are done
 #pragma unroll
-        for (int i = 0; i < 2; ++i) half_insert<W>(K, C, key[p][i], fr[p][i]);
+        for (uint32_t i = 0; i < kHalfCap; i += 32) T[i + hl] = 0ull;
+        if (hl == 0) t.best[hf] = 0ull;
         wave_lds_sync();
-        uint64_t best = 0ull;
-        for (uint32_t i = hl; i < kHalfCap; i += 32) {
-            const uint32_t kk = K[i];
-            if (kk) best = max_u64(best, H::label_score(a, v[p], kk, C[i]));
+        // per half, the lanes holding the half's first pending label merge
+        // into one insert by the lowest of them (a converged vertex shows one
+        // label on most edges, and same-address LDS atomics serialise); the
+        // merged lanes neither insert nor score (the leader's slot holds the
+        // same count)
+        uint32_t hs[2], sk[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            uint32_t k = key[p][i], f = W ? fr[p][i] : 1u;
+            const unsigned long long rest = __ballot(k != 0u);
+            const uint32_t rlo = (uint32_t)rest, rhi = (uint32_t)(rest >> 32);
+            const int flo = rlo ? (int)__builtin_ctz(rlo) : 0;
+            const int fhi = rhi ? 32 + (int)__builtin_ctz(rhi) : 32;
+            const uint32_t llo = rlo ? (uint32_t)__builtin_amdgcn_readlane((int)k, flo) : 0u;
+            const uint32_t lhi = rhi ? (uint32_t)__builtin_amdgcn_readlane((int)k, fhi) : 0u;
+            const uint32_t lead = hf ? lhi : llo;
+            const bool same = k != 0u && k == lead;
+            const unsigned long long sm = __ballot(same);
+            const uint32_t cnt = (uint32_t)__popc(hf ? (uint32_t)(sm >> 32) : (uint32_t)sm);
+            if (W) {
+                // weighted: the half's sum of the merged lanes' frequencies
+                uint32_t x = same ? f : 0u;
+#pragma unroll
+                for (int d = 16; d >= 1; d >>= 1) x += shfl_u32(x, lane ^ d);
+                if (same && cnt >= 2u) f = x;
+            } else if (same && cnt >= 2u) {
+                f = cnt;
+            }
+            if (same && cnt >= 2u && lane != (hf ? fhi : flo)) k = 0u;  // merged into the leader
+            sk[i] = k;
+            hs[i] = k ? kc_insert(T, kHalfCap - 1u, k, f) : 0u;
         }
-        best = half_max_u64(best);
-        wave_lds_sync();  // the next pair clears what this scan read
+        wave_lds_sync();
+        // one score per distinct label: the slot's creator
+        uint64_t pm = 0ull;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            if (sk[i] && (hs[i] & kClaimed))
+                pm = max_u64(pm, H::label_score(a, v[p], sk[i], (uint32_t)T[hs[i] & ~kClaimed]));
+        if (pm) atomicMax(&t.best[hf], (unsigned long long)pm);
+        wave_lds_sync();
+        const uint64_t best = t.best[hf];
         bool changed = false;
         if (hl == 0 && run[p]) {
             const int32_t nl = best ? packed_label(best) : old[p];
@@ -1070,7 +1113,9 @@ int launch_multi(bool weighted, const JobDesc* jobs, const int32_t* cum, int32_t
 // Workgroups each launch of a folded step needs for one job.
 void job_blocks(const Plan& p, int32_t* blocks) {
     for (int L = 0; L < kLaunches; ++L) blocks[L] = 0;
-    blocks[kLaunchPacked] = (int32_t)((p.groups.size() + 4 * kPackNG - 1) / (4 * kPackNG));
+    // folded jobs run the layered hooks, which skip edge-less groups
+    static_assert(LayeredHooks::kSkipIsolated, "folded launches run groups with edges only");
+    blocks[kLaunchPacked] = (int32_t)((p.groups_with_edges + 4 * kPackNG - 1) / (4 * kPackNG));
     blocks[kLaunchHalf] = (int32_t)((p.half.size() + 8 * kHalfPairs - 1) / (8 * kHalfPairs));
     blocks[kLaunchWave] = (int32_t)((p.wave.size() + 4 * kWaveNV - 1) / (4 * kWaveNV));
     for (int c = 0; c < p.nclasses; ++c) blocks[kLaunchBlock0 + c] = (int32_t)p.block[c].size();
@@ -1154,6 +1199,53 @@ __global__ void k_fill_i32(int32_t* p, int32_t value, int64_t count) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
         p[i] = value;
 }
+// Layered LP's volume V(l) = #{v : L[v] = l} of the labels a step wrote
+// (header/kernel.cuh:2626's gcounter_temp, counted in a pass of its own).
+// Once labels converge a few labels hold most vertices, and one atomic per
+// vertex on their counters serialises in L2.  Each wave walks kVolRows rows
+// of 64 consecutive vertices and keeps one label of its own (the most common
+// of its first row's labels) in a register count, flushed once; in every
+// row the remaining lanes merge their first label into one atomic and the
+// rest add one each.
+constexpr int kVolRows = 64;
+__global__ __launch_bounds__(256) void k_volume(const int32_t* __restrict__ L, int32_t* __restrict__ V,
+                                                int32_t n) {
+    const int lane = lane_id();
+    const int64_t v0 = (((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6) * (64 * kVolRows);
+    if (v0 >= n) return;  // wave-uniform
+    int32_t keep_l = -1;   // wave-uniform kept label and its count
+    uint32_t keep_c = 0u;
+    for (int r = 0; r < kVolRows && v0 + (int64_t)r * 64 < n; ++r) {
+        const int64_t v = v0 + (int64_t)r * 64 + lane;
+        int32_t l = v < n ? L[v] : -1;
+        if (r == 0) {
+            // the most common of up to three labels of the first row
+            int32_t x = l;
+            for (int k = 0; k < 3; ++k) {
+                const unsigned long long act = __ballot(x >= 0);
+                if (act == 0ull) break;
+                const int32_t lead = __builtin_amdgcn_readlane(x, (int)__builtin_ctzll(act));
+                const uint32_t c = (uint32_t)__popcll(__ballot(x == lead));
+                if (c > keep_c) {
+                    keep_c = c;
+                    keep_l = lead;
+                }
+                if (x == lead) x = -1;
+            }
+            keep_c = 0u;
+        }
+        keep_c += (uint32_t)__popcll(__ballot(l == keep_l));
+        if (l == keep_l) l = -1;
+        const unsigned long long act = __ballot(l >= 0);
+        if (act == 0ull) continue;
+        const int32_t lead = __builtin_amdgcn_readlane(l, (int)__builtin_ctzll(act));
+        const unsigned long long sm = __ballot(l == lead);
+        if (lane == (int)__builtin_ctzll(act)) atomicAdd(&V[lead], (int32_t)__popcll(sm));
+        else if (l >= 0 && l != lead) atomicAdd(&V[l], 1);
+    }
+    if (keep_c && lane == 0) atomicAdd(&V[keep_l], (int32_t)keep_c);
+}
+
 // Seeded LP's L_0 (header/kernel.cuh:453-466): v for a seed, -1 otherwise.
 __global__ void k_seed_labels(int32_t* p, const uint8_t* __restrict__ seeds, int64_t count) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -1379,6 +1471,14 @@ int launch_gather(const StepArgs& a, int64_t m, int32_t* adj, hipStream_t s) {
         hipLaunchKernelGGL(k_gather, dim3(grid), dim3(256), 0, s, a.nbr, a.L, adj, m);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "gather launch: %s", hipGetErrorString(e));
+}
+
+int launch_volume(const int32_t* L, int32_t* V, int32_t n, hipStream_t s) {
+    if (n <= 0) return GLP_OK;
+    const int64_t waves = ((int64_t)n + 64 * kVolRows - 1) / (64 * kVolRows);
+    hipLaunchKernelGGL(k_volume, dim3(grid_for(waves, 4)), dim3(256), 0, s, L, V, n);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "volume launch: %s", hipGetErrorString(e));
 }
 
 int launch_seed_labels(int32_t* p, const uint8_t* seeds, int64_t count, hipStream_t s) {

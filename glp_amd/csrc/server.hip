@@ -183,7 +183,7 @@ int glp_server::tick() {
         }
         weighted |= j->p.weighted != 0;
         d.groups = dg.d_groups;
-        d.ngroups = (int32_t)p.groups.size();
+        d.ngroups = (int32_t)p.groups_with_edges;  // layered hooks skip edge-less groups
         d.ngroups_e = (int32_t)p.groups_with_edges;
         d.half = dg.d_half;
         d.nhalf = (int32_t)p.half.size();
@@ -217,6 +217,9 @@ int glp_server::tick() {
                            stream));
     int rc = launch_multi(weighted, d_jobs, d_cum, nj, grid, tune, class_cap, stream);
     if (rc != GLP_OK) return rc;
+    for (int32_t k = 0; k < nj; ++k)
+        if (h_jobs[k].a.Vw && (rc = launch_volume(h_jobs[k].a.Lw, h_jobs[k].a.Vw, h_jobs[k].a.n, stream)))
+            return rc;
     GLP_HIP(hipStreamSynchronize(stream));
     for (auto* j : act) {
         j->cur ^= 1;

@@ -436,6 +436,7 @@ int enqueue_step(glp_session* s, hipEvent_t* ev) {
     if ((rc = launch_step(s->method, s->p.weighted != 0, s->p.gather != 0, *s->g->dev, s->hs, a,
                           s->stream, ev)))
         return rc;
+    if (s->method == Method::Layered && (rc = launch_volume(a.Lw, a.Vw, s->n, s->stream))) return rc;
     if ((rc = launch_end_step(s->d_acc, s->d_ring, s->d_ctr, kChangeRing, s->stream, s->fctl))) return rc;
     if (s->method != Method::Slpa) s->cur ^= 1;
     s->t += 1;

@@ -365,11 +365,17 @@ extern "C" double orc_cpulp_run(int32_t n, const int32_t* off, const int32_t* nb
 }
 
 extern "C" double orc_cpulp_sample(int32_t n, const int32_t* off, const int32_t* nbr, int32_t* labels,
-                                   const int32_t* verts, int64_t nverts, int32_t threads) {
-    (void)n;
+                                   const int32_t* verts, int64_t nverts, int32_t threads,
+                                   int32_t balanced) {
     std::vector<int32_t> prelabels(labels, labels + n);
+    // balanced = 0: the reference's `omp parallel for` with the default
+    // (static) schedule over vertices in id order (tutorial/CPUlp.cuh:50-51),
+    // so with degree-sorted ids the first thread gets the hubs, as in a whole
+    // step; balanced = 1: dynamic chunks of 64 vertices
+    if (balanced) omp_set_schedule(omp_sched_dynamic, 64);
+    else omp_set_schedule(omp_sched_static, 0);
     auto t0 = std::chrono::system_clock::now();
-#pragma omp parallel for num_threads(threads) schedule(dynamic, 64)
+#pragma omp parallel for num_threads(threads) schedule(runtime)
     for (int64_t k = 0; k < nverts; ++k) {
         const int32_t u = verts[k];
         std::map<int32_t, int> label_count;

@@ -79,9 +79,11 @@ double orc_cpulp_run(int32_t n, const int32_t* off, const int32_t* nbr, int32_t 
 
 /* One verbatim CPUlp step (tutorial/CPUlp.cuh:47-81) restricted to the
  * vertices in verts[0..nverts): the bounded CPU-baseline sample.  labels is
- * read and updated in place as the reference does; returns seconds. */
+ * read and updated in place as the reference does; balanced = 0 keeps the
+ * reference's static OpenMP schedule, 1 uses dynamic chunks; returns
+ * seconds. */
 double orc_cpulp_sample(int32_t n, const int32_t* off, const int32_t* nbr, int32_t* labels,
-                        const int32_t* verts, int64_t nverts, int32_t threads);
+                        const int32_t* verts, int64_t nverts, int32_t threads, int32_t balanced);
 
 #ifdef __cplusplus
 }

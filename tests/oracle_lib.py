@@ -202,13 +202,15 @@ def csr_from_edges(n, edges, weights=None):
 
 
 _s("orc_cpulp_sample", ctypes.c_double, ctypes.c_int32, _i32p, _i32p, _i32p, _i32p, ctypes.c_int64,
-   ctypes.c_int32)
+   ctypes.c_int32, ctypes.c_int32)
 
 
-def cpulp_sample(off, nbr, labels, verts, threads):
-    """Verbatim CPUlp step over `verts` only; labels updated in place; seconds."""
+def cpulp_sample(off, nbr, labels, verts, threads, balanced=False):
+    """Verbatim CPUlp step over `verts` only (the reference's static OpenMP
+    schedule, or dynamic chunks when balanced); labels updated in place;
+    seconds."""
     off, nbr, _ = _csr(off, nbr, None)
     verts = np.ascontiguousarray(verts, dtype=np.int32)
     assert labels.dtype == np.int32 and labels.flags.c_contiguous
     return _o.orc_cpulp_sample(len(off) - 1, _i(off), _i(nbr), _i(labels), _i(verts), len(verts),
-                               threads)
+                               threads, int(balanced))
