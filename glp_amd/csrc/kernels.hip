@@ -35,24 +35,25 @@ __device__ __forceinline__ T lds_load(const T* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Inserts (key, f) into an LDS open-addressing table of `mask+1` slots.
-// The table always has a free slot: its size is at least twice the number of
-// edges inserted into it.
-__device__ __forceinline__ void lds_insert(uint32_t* K, uint32_t* C, uint32_t mask, uint32_t key,
-                                           uint32_t f) {
+// Inserts (key, f) into an LDS table of (key << 32 | frequency sum) slots
+// and returns the slot, with kClaimed set when this insert created it.  A
+// key claims a free slot with its first frequency already in it; later ones
+// add to the low word (sums stay below 2^31, so they never carry into the
+// key).  The table always has a free slot.
+constexpr uint32_t kClaimed = 0x80000000u;
+__device__ __forceinline__ uint32_t kc_insert(unsigned long long* T, uint32_t mask, uint32_t key,
+                                              uint32_t f) {
     uint32_t h = fmix32(key) & mask;
+    const unsigned long long mine = (unsigned long long)key << 32;
     for (;;) {
-        uint32_t k = lds_load(&K[h]);
-        if (k == key) {
-            atomicAdd(&C[h], f);
-            return;
+        unsigned long long cur = lds_load(&T[h]);
+        if (cur == 0ull) {
+            cur = atomicCAS(&T[h], 0ull, mine | f);
+            if (cur == 0ull) return h | kClaimed;
         }
-        if (k == 0u) {
-            uint32_t prev = atomicCAS(&K[h], 0u, key);
-            if (prev == 0u || prev == key) {
-                atomicAdd(&C[h], f);
-                return;
-            }
+        if ((uint32_t)(cur >> 32) == key) {
+            atomicAdd(&T[h], (unsigned long long)f);
+            return h;
         }
         h = (h + 1u) & mask;
     }
@@ -66,7 +67,7 @@ __device__ __forceinline__ void lds_insert(uint32_t* K, uint32_t* C, uint32_t ma
 // the table's atomics do, so the counts are unchanged).
 constexpr int kPeels = 2;
 template <bool W>
-__device__ __forceinline__ void wave_insert(uint32_t* Kt, uint32_t* Ct, uint32_t mask, uint32_t key,
+__device__ __forceinline__ void wave_insert(unsigned long long* T, uint32_t mask, uint32_t key,
                                             uint32_t f) {
     const int lane = lane_id();
     uint32_t ik = key, iff = f;  // what this lane finally inserts
@@ -86,7 +87,7 @@ __device__ __forceinline__ void wave_insert(uint32_t* Kt, uint32_t* Ct, uint32_t
             key = 0u;
         }
     }
-    if (ik) lds_insert(Kt, Ct, mask, ik, iff);
+    if (ik) kc_insert(T, mask, ik, iff);
 }
 
 // Counts edges [b, e) of vertex v into an LDS table, STRIDE threads
@@ -96,7 +97,7 @@ __device__ __forceinline__ void wave_insert(uint32_t* Kt, uint32_t* Ct, uint32_t
 // instead of one.  The loop is wave-uniform (wave_insert needs every lane).
 template <class H, bool W, bool G, int K, int STRIDE>
 __device__ __forceinline__ void count_edges(const StepArgs& a, int32_t v, int32_t b, int32_t e,
-                                            int tid, uint32_t* Kt, uint32_t* Ct, uint32_t mask) {
+                                            int tid, unsigned long long* T, uint32_t mask) {
     static_assert(STRIDE % 64 == 0, "whole waves");
     const int lane = tid & 63;
     for (int32_t jw = b + (tid & ~63); jw < e; jw += STRIDE * K) {
@@ -114,7 +115,7 @@ __device__ __forceinline__ void count_edges(const StepArgs& a, int32_t v, int32_
         for (int i = 0; i < K; ++i)
             if (jw + lane + i * STRIDE >= e) key[i] = 0u;
 #pragma unroll
-        for (int i = 0; i < K; ++i) wave_insert<W>(Kt, Ct, mask, key[i], fr[i]);
+        for (int i = 0; i < K; ++i) wave_insert<W>(T, mask, key[i], fr[i]);
     }
 }
 
@@ -364,30 +365,6 @@ __device__ __forceinline__ void packed_body(const StepArgs& a, const int2* __res
 constexpr int kHalfPairs = 2;
 constexpr uint32_t kHalfCap = 2 * kHalfMaxDeg;
 
-// Inserts (key, f) into an LDS table of (key << 32 | frequency sum) slots
-// and returns the slot, with kClaimed set when this insert created it.  A
-// key claims a free slot with its first frequency already in it; later ones
-// add to the low word (sums stay below 2^31, so they never carry into the
-// key).  The table always has a free slot.
-constexpr uint32_t kClaimed = 0x80000000u;
-__device__ __forceinline__ uint32_t kc_insert(unsigned long long* T, uint32_t mask, uint32_t key,
-                                              uint32_t f) {
-    uint32_t h = fmix32(key) & mask;
-    const unsigned long long mine = (unsigned long long)key << 32;
-    for (;;) {
-        unsigned long long cur = lds_load(&T[h]);
-        if (cur == 0ull) {
-            cur = atomicCAS(&T[h], 0ull, mine | f);
-            if (cur == 0ull) return h | kClaimed;
-        }
-        if ((uint32_t)(cur >> 32) == key) {
-            atomicAdd(&T[h], (unsigned long long)f);
-            return h;
-        }
-        h = (h + 1u) & mask;
-    }
-}
-
 // Per-wave LDS of the half tier: one (key << 32 | sum) table per half and
 // each half's best packed score.
 struct HalfLds {
@@ -504,8 +481,7 @@ constexpr int kWaveNV = 4;
 template <class H, bool W, bool G>
 __device__ __forceinline__ void wave_body(const StepArgs& a, const int32_t* __restrict__ verts,
                                           int32_t nv, uint32_t blk) {
-    __shared__ uint32_t s_key[4][kWaveCap];
-    __shared__ uint32_t s_cnt[4][kWaveCap];
+    __shared__ unsigned long long s_tab[4][kWaveCap];  // (label key << 32 | count) slots
     const int lane = lane_id(), wv = threadIdx.x >> 6;
     const int32_t i0 = ((int32_t)blk * 4 + wv) * kWaveNV;
     if (i0 >= nv) return;  // wave-uniform
@@ -531,26 +507,22 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int32_t* __re
             key[k][i] = H::template load_neighbor<W, G>(a, v[k], run[k] ? min(j, e[k] - 1) : b[k], &fr[k][i]);
             if (j >= e[k]) key[k][i] = 0u;
         }
-    uint32_t* K = s_key[wv];
-    uint32_t* C = s_cnt[wv];
+    unsigned long long* T = s_tab[wv];
     uint32_t nch = 0u;  // changed labels (lane 0)
 #pragma unroll
     for (int k = 0; k < kWaveNV; ++k) {
         if (!run[k]) continue;  // wave-uniform
         const uint32_t cap = min(max(next_pow2(2u * (uint32_t)(e[k] - b[k])), 64u), (uint32_t)kWaveCap);
         const uint32_t mask = cap - 1u;
-        for (uint32_t i = lane; i < cap; i += 64) {
-            K[i] = 0u;
-            C[i] = 0u;
-        }
+        for (uint32_t i = lane; i < cap; i += 64) T[i] = 0ull;
         wave_lds_sync();
 #pragma unroll
-        for (int i = 0; i < 4; ++i) wave_insert<W>(K, C, mask, key[k][i], fr[k][i]);
+        for (int i = 0; i < 4; ++i) wave_insert<W>(T, mask, key[k][i], fr[k][i]);
         wave_lds_sync();
         uint64_t best = 0ull;
         for (uint32_t i = lane; i < cap; i += 64) {
-            const uint32_t kk = K[i];
-            if (kk) best = max_u64(best, H::label_score(a, v[k], kk, C[i]));
+            const unsigned long long x = T[i];
+            if (x) best = max_u64(best, H::label_score(a, v[k], (uint32_t)(x >> 32), (uint32_t)x));
         }
         best = wave_max_u64(best);
         wave_lds_sync();  // the next vertex clears what this scan read
@@ -584,27 +556,23 @@ __device__ __forceinline__ uint64_t block_max_u64(uint64_t x, unsigned long long
 template <class H, bool W, bool G, int NT>
 __device__ __forceinline__ void block_body(const StepArgs& a, const int32_t* __restrict__ verts,
                                            uint32_t cap_max, uint32_t blk) {
-    extern __shared__ uint32_t s_dyn[];
+    extern __shared__ __align__(16) uint32_t s_dyn[];
     __shared__ unsigned long long s_red[NT / 64];
     __shared__ uint32_t s_changed;
-    uint32_t* K = s_dyn;
-    uint32_t* C = s_dyn + cap_max;
+    unsigned long long* T = reinterpret_cast<unsigned long long*>(s_dyn);  // cap_max slots
     const int32_t v = verts[blk];
     if (!is_active<H>(a, v) || is_fixed<H>(a, v)) return;  // block-uniform
     const int32_t b = a.off[v], e = a.off[v + 1];
     const uint32_t cap = min(max(next_pow2(2u * (uint32_t)(e - b)), 64u), cap_max);
     const uint32_t mask = cap - 1u;
-    for (uint32_t i = threadIdx.x; i < cap; i += NT) {
-        K[i] = 0u;
-        C[i] = 0u;
-    }
+    for (uint32_t i = threadIdx.x; i < cap; i += NT) T[i] = 0ull;
     __syncthreads();
-    count_edges<H, W, G, 8, NT>(a, v, b, e, threadIdx.x, K, C, mask);
+    count_edges<H, W, G, 8, NT>(a, v, b, e, threadIdx.x, T, mask);
     __syncthreads();
     uint64_t best = 0ull;
     for (uint32_t i = threadIdx.x; i < cap; i += NT) {
-        const uint32_t k = K[i];
-        if (k) best = max_u64(best, H::label_score(a, v, k, C[i]));
+        const unsigned long long x = T[i];
+        if (x) best = max_u64(best, H::label_score(a, v, (uint32_t)(x >> 32), (uint32_t)x));
     }
     best = block_max_u64<NT>(best, s_red);
     if (threadIdx.x < 64) {
@@ -664,11 +632,10 @@ template <class H, bool W, bool G>
 __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
     // dynamic LDS: table of next_pow2(2*chunk) keys and counts, then the
     // bucket histogram (kMaxBuckets counters)
-    extern __shared__ uint32_t s_dyn[];
+    extern __shared__ __align__(16) uint32_t s_dyn[];
     __shared__ uint32_t s_npairs;
     const uint32_t cap_max = next_pow2(2u * (uint32_t)h.chunk);
-    uint32_t* K = s_dyn;
-    uint32_t* C = s_dyn + cap_max;
+    unsigned long long* T = reinterpret_cast<unsigned long long*>(s_dyn);  // cap_max slots
     uint32_t* s_hist = s_dyn + 2 * cap_max;
     const int2 hc = h.chunks[blk];
     const int32_t v = h.hubs[hc.x];
@@ -685,20 +652,18 @@ __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h
     const int32_t e = min(b + h.chunk, ve);
     const uint32_t cap = min(max(next_pow2(2u * (uint32_t)(e - b)), 64u), cap_max);
     const uint32_t mask = cap - 1u;
-    for (uint32_t i = threadIdx.x; i < cap; i += kHubThreads) {
-        K[i] = 0u;
-        C[i] = 0u;
-    }
+    for (uint32_t i = threadIdx.x; i < cap; i += kHubThreads) T[i] = 0ull;
     for (uint32_t q = threadIdx.x; q < P; q += kHubThreads) s_hist[q] = 0u;
     if (threadIdx.x == 0) s_npairs = 0u;
     __syncthreads();
-    count_edges<H, W, G, 8, kHubThreads>(a, v, b, e, threadIdx.x, K, C, mask);
+    count_edges<H, W, G, 8, kHubThreads>(a, v, b, e, threadIdx.x, T, mask);
     __syncthreads();
     unsigned long long* out = h.hs.pairs + h.base[hc.x] + (int64_t)hc.y * h.chunk;
     // cap is a power of two >= 64, so each wave runs this loop whole; one
     // LDS atomic per wave reserves its pair slots
     for (uint32_t i = threadIdx.x; i < cap; i += kHubThreads) {
-        const uint32_t k = K[i];
+        const unsigned long long x = T[i];  // (label key << 32 | count): the pair itself
+        const uint32_t k = (uint32_t)(x >> 32);
         const unsigned long long bal = __ballot(k != 0u);
         if (bal == 0ull) continue;
         uint32_t first = 0u;
@@ -707,7 +672,7 @@ __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h
         if (!k) continue;
         const uint32_t slot = first + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-        out[slot] = ((unsigned long long)k << 32) | C[i];
+        out[slot] = x;
         atomicAdd(&s_hist[bucket_of(fmix32(k), P)], 1u);
     }
     __syncthreads();
@@ -776,7 +741,7 @@ __device__ __forceinline__ void hub_scatter_body(const StepArgs& a, const HubArg
 // table.  Stops early (the caller halves the range) once more than `limit`
 // distinct keys arrived; the table has room for limit + kHubThreads.
 __device__ __forceinline__ void part_pass(const unsigned long long* pr, uint32_t n, uint64_t lo,
-                                          uint64_t hi, uint32_t* K, uint32_t* C, uint32_t shift,
+                                          uint64_t hi, unsigned long long* T, uint32_t shift,
                                           uint32_t limit, uint32_t* s_distinct, uint32_t* s_over) {
     const uint32_t mask = (1u << (32 - shift)) - 1u;
     constexpr int kU = 4;  // pairs loaded per thread before any is inserted
@@ -792,25 +757,20 @@ __device__ __forceinline__ void part_pass(const unsigned long long* pr, uint32_t
             const uint32_t key = (uint32_t)(xs[u] >> 32);
             const uint32_t hk = fmix32(key);
             if (hk < lo || hk >= hi) continue;
-            const uint32_t f = (uint32_t)xs[u];
+            // a pair is already a (key << 32 | count) slot value
             uint32_t s = (hk * 0x9e3779b1u) >> shift;
             for (;;) {
-                const uint32_t k = lds_load(&K[s]);
-                if (k == key) {
-                    atomicAdd(&C[s], f);
-                    break;
-                }
-                if (k == 0u) {
-                    const uint32_t prev = atomicCAS(&K[s], 0u, key);
-                    if (prev == 0u) {
+                unsigned long long cur = lds_load(&T[s]);
+                if (cur == 0ull) {
+                    cur = atomicCAS(&T[s], 0ull, xs[u]);
+                    if (cur == 0ull) {
                         if (atomicAdd(s_distinct, 1u) + 1u > limit) atomicExch(s_over, 1u);
-                        atomicAdd(&C[s], f);
                         break;
                     }
-                    if (prev == key) {
-                        atomicAdd(&C[s], f);
-                        break;
-                    }
+                }
+                if ((uint32_t)(cur >> 32) == key) {
+                    atomicAdd(&T[s], xs[u] & 0xffffffffull);
+                    break;
                 }
                 s = (s + 1u) & mask;
             }
@@ -846,7 +806,7 @@ template <class H>
 __device__ __forceinline__ void hub_part_count(const StepArgs& a, const HubArgs& h, const PartJob& j,
                                                unsigned long long* s_red, uint32_t* s_distinct,
                                                uint32_t* s_over) {
-    extern __shared__ uint32_t s_dyn[];
+    extern __shared__ __align__(16) uint32_t s_dyn[];
     const unsigned long long* pr = h.hs.sorted + h.base[j.hub] + j.start;
     const uint32_t n = j.stop - j.start;
     const uint32_t cap_max = next_pow2(4u * (uint32_t)h.part_keys);
@@ -856,8 +816,7 @@ __device__ __forceinline__ void hub_part_count(const StepArgs& a, const HubArgs&
     uint32_t cap = cap_max;
     if (h.part_limit == 0 && n <= cap_max / 2u) cap = max(next_pow2(2u * n), 64u);
     const uint32_t shift = 32u - (31u - __builtin_clz(cap));
-    uint32_t* K = s_dyn;
-    uint32_t* C = s_dyn + cap;
+    unsigned long long* T = reinterpret_cast<unsigned long long*>(s_dyn);  // cap slots
     const uint64_t end = bucket_lo(j.q + 1, j.P);
     uint64_t lo = bucket_lo(j.q, j.P), hi = end;
     const uint32_t limit = h.part_limit > 0 ? (uint32_t)h.part_limit : cap / 2u;
@@ -868,7 +827,7 @@ __device__ __forceinline__ void hub_part_count(const StepArgs& a, const HubArgs&
             *s_over = 0u;
         }
         __syncthreads();
-        part_pass(pr, n, lo, hi, K, C, shift, limit, s_distinct, s_over);
+        part_pass(pr, n, lo, hi, T, shift, limit, s_distinct, s_over);
         __syncthreads();
         if (*s_over != 0u) {
             // the pass stopped part-way: clear the table and retry the lower
@@ -880,11 +839,10 @@ __device__ __forceinline__ void hub_part_count(const StepArgs& a, const HubArgs&
             continue;
         }
         for (uint32_t i = threadIdx.x; i < cap; i += kHubThreads) {
-            const uint32_t k = K[i];
-            if (k) {
-                best = max_u64(best, H::label_score(a, j.v, k, C[i]));
-                K[i] = 0u;
-                C[i] = 0u;
+            const unsigned long long x = T[i];
+            if (x) {
+                best = max_u64(best, H::label_score(a, j.v, (uint32_t)(x >> 32), (uint32_t)x));
+                T[i] = 0ull;
             }
         }
         lo = hi;
@@ -903,7 +861,7 @@ constexpr int kPartBatch = kHubThreads;
 template <class H>
 __device__ __forceinline__ void hub_part_batch_body(const StepArgs& a, const HubArgs& h,
                                                     uint32_t blk) {
-    extern __shared__ uint32_t s_dyn[];
+    extern __shared__ __align__(16) uint32_t s_dyn[];
     __shared__ PartJob s_jobs[kPartBatch];
     __shared__ unsigned long long s_red[kHubThreads / 64];
     __shared__ uint32_t s_n, s_distinct, s_over;
@@ -1201,49 +1159,59 @@ __global__ void k_fill_i32(int32_t* p, int32_t value, int64_t count) {
 }
 // Layered LP's volume V(l) = #{v : L[v] = l} of the labels a step wrote
 // (header/kernel.cuh:2626's gcounter_temp, counted in a pass of its own).
-// Once labels converge a few labels hold most vertices, and one atomic per
-// vertex on their counters serialises in L2.  Each wave walks kVolRows rows
-// of 64 consecutive vertices and keeps one label of its own (the most common
-// of its first row's labels) in a register count, flushed once; in every
-// row the remaining lanes merge their first label into one atomic and the
-// rest add one each.
-constexpr int kVolRows = 64;
-__global__ __launch_bounds__(256) void k_volume(const int32_t* __restrict__ L, int32_t* __restrict__ V,
-                                                int32_t n) {
+// While labels consolidate, a few labels each hold millions of vertices
+// spread over the id range, and one global atomic per vertex on their
+// counters serialises in L2.  A workgroup counts a chunk of kVolChunk
+// vertices in an LDS table of (label+1, count) slots (each wave first merges
+// its row's first label) and flushes one atomic per distinct label; a label
+// that finds no slot within kVolProbe probes goes to global memory directly.
+constexpr int kVolThreads = 256;
+constexpr int kVolChunk = 16384;
+constexpr uint32_t kVolSlots = 2048;
+constexpr int kVolProbe = 8;
+__global__ __launch_bounds__(kVolThreads) void k_volume(const int32_t* __restrict__ L,
+                                                        int32_t* __restrict__ V, int32_t n) {
+    __shared__ unsigned long long T[kVolSlots];
     const int lane = lane_id();
-    const int64_t v0 = (((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6) * (64 * kVolRows);
-    if (v0 >= n) return;  // wave-uniform
-    int32_t keep_l = -1;   // wave-uniform kept label and its count
-    uint32_t keep_c = 0u;
-    for (int r = 0; r < kVolRows && v0 + (int64_t)r * 64 < n; ++r) {
-        const int64_t v = v0 + (int64_t)r * 64 + lane;
-        int32_t l = v < n ? L[v] : -1;
-        if (r == 0) {
-            // the most common of up to three labels of the first row
-            int32_t x = l;
-            for (int k = 0; k < 3; ++k) {
-                const unsigned long long act = __ballot(x >= 0);
-                if (act == 0ull) break;
-                const int32_t lead = __builtin_amdgcn_readlane(x, (int)__builtin_ctzll(act));
-                const uint32_t c = (uint32_t)__popcll(__ballot(x == lead));
-                if (c > keep_c) {
-                    keep_c = c;
-                    keep_l = lead;
-                }
-                if (x == lead) x = -1;
-            }
-            keep_c = 0u;
-        }
-        keep_c += (uint32_t)__popcll(__ballot(l == keep_l));
-        if (l == keep_l) l = -1;
+    for (uint32_t i = threadIdx.x; i < kVolSlots; i += kVolThreads) T[i] = 0ull;
+    __syncthreads();
+    const int64_t c0 = (int64_t)blockIdx.x * kVolChunk;
+    const int64_t c1 = min((int64_t)n, c0 + kVolChunk);
+    for (int64_t r = c0 + (threadIdx.x & ~63); r < c1; r += kVolThreads) {  // wave-uniform rows
+        const int64_t v = r + lane;
+        int32_t l = v < c1 ? L[v] : -1;
+        uint32_t c = 1u;
         const unsigned long long act = __ballot(l >= 0);
-        if (act == 0ull) continue;
-        const int32_t lead = __builtin_amdgcn_readlane(l, (int)__builtin_ctzll(act));
+        const int first = (int)__builtin_ctzll(act);
+        const int32_t lead = __builtin_amdgcn_readlane(l, first);
         const unsigned long long sm = __ballot(l == lead);
-        if (lane == (int)__builtin_ctzll(act)) atomicAdd(&V[lead], (int32_t)__popcll(sm));
-        else if (l >= 0 && l != lead) atomicAdd(&V[l], 1);
+        if (l == lead) {
+            if (lane == first) c = (uint32_t)__popcll(sm);
+            else l = -1;  // merged into the first lane
+        }
+        if (l < 0) continue;
+        const uint32_t key = (uint32_t)l + 1u;
+        uint32_t h = fmix32(key) & (kVolSlots - 1u);
+        bool done = false;
+        for (int k = 0; k < kVolProbe && !done; ++k) {
+            unsigned long long cur = lds_load(&T[h]);
+            if (cur == 0ull) {
+                cur = atomicCAS(&T[h], 0ull, ((unsigned long long)key << 32) | c);
+                if (cur == 0ull) done = true;
+            }
+            if (!done && (uint32_t)(cur >> 32) == key) {
+                atomicAdd(&T[h], (unsigned long long)c);
+                done = true;
+            }
+            h = (h + 1u) & (kVolSlots - 1u);
+        }
+        if (!done) atomicAdd(&V[l], (int32_t)c);
     }
-    if (keep_c && lane == 0) atomicAdd(&V[keep_l], (int32_t)keep_c);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kVolSlots; i += kVolThreads) {
+        const unsigned long long x = T[i];
+        if (x) atomicAdd(&V[(uint32_t)(x >> 32) - 1u], (int32_t)(uint32_t)x);
+    }
 }
 
 // Seeded LP's L_0 (header/kernel.cuh:453-466): v for a seed, -1 otherwise.
@@ -1475,8 +1443,7 @@ int launch_gather(const StepArgs& a, int64_t m, int32_t* adj, hipStream_t s) {
 
 int launch_volume(const int32_t* L, int32_t* V, int32_t n, hipStream_t s) {
     if (n <= 0) return GLP_OK;
-    const int64_t waves = ((int64_t)n + 64 * kVolRows - 1) / (64 * kVolRows);
-    hipLaunchKernelGGL(k_volume, dim3(grid_for(waves, 4)), dim3(256), 0, s, L, V, n);
+    hipLaunchKernelGGL(k_volume, dim3(grid_for(n, kVolChunk)), dim3(kVolThreads), 0, s, L, V, n);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "volume launch: %s", hipGetErrorString(e));
 }
