@@ -78,6 +78,9 @@ struct ClassicHooks {
     static constexpr bool kSkipIsolated = true;
     // a vertex whose neighbours all kept their labels keeps its own: exact
     static constexpr bool kFrontier = true;
+    // the score is the frequency sum, so a label holding more than half of a
+    // vertex's counted frequency wins outright
+    static constexpr bool kScoreIsCount = true;
     __device__ static __forceinline__ int32_t pick_label(const StepArgs& a, int32_t vid) {
         return a.keep[vid];
     }
@@ -142,6 +145,7 @@ struct SlpaHooks {
     static constexpr bool kPick = true;
     static constexpr bool kSkipIsolated = false;  // every memory row is written
     static constexpr bool kFrontier = false;      // speakers change every step
+    static constexpr bool kScoreIsCount = true;
     // Speaker rule: u shows M_u[R(seed,t,u) mod t] (memory rows 0..t-1).
     __device__ static __forceinline__ int32_t pick_label(const StepArgs& a, int32_t vid) {
         uint64_t x = (a.seed << 40) ^ ((uint64_t)(uint32_t)a.t << 32) ^ (uint64_t)(uint32_t)vid;
@@ -180,6 +184,7 @@ struct SeededHooks {
     static constexpr bool kSkipIsolated = true;  // isolated vertices keep their label
     static constexpr bool kFrontier = false;
     static constexpr bool kSeeded = true;        // seeds are never recomputed
+    static constexpr bool kScoreIsCount = true;
     template <bool W, bool G>
     __device__ static __forceinline__ uint32_t load_neighbor(const StepArgs& a, int32_t vid,
                                                              int32_t j, uint32_t* freq) {
@@ -195,6 +200,13 @@ struct SeededHooks {
         a.Lw[vid] = label;
     }
 };
+
+// Hook sets without kScoreIsCount = true get no majority shortcut.
+template <class H, class = void>
+struct count_score_of : std::false_type {};
+template <class H>
+struct count_score_of<H, std::void_t<decltype(H::kScoreIsCount)>>
+    : std::bool_constant<H::kScoreIsCount> {};
 
 // Hook sets without a kSeeded member have no fixed vertices.
 template <class H, class = void>

@@ -183,7 +183,9 @@ __device__ __forceinline__ void mark_vertex(const StepArgs& a, int32_t v, int32_
 // table of jobs folded into one launch (blk = index inside the owning job).
 
 // ---------------------------------------------------------------- packed ----
-// A wave takes kPackNG consecutive groups.  When all of them have edges (the
+// A wave takes NG consecutive groups (NG chosen per launch from the amount of
+// work: more groups per wave give each wave more loads in flight, fewer
+// leave small launches more waves).  When all of them have edges (the
 // planner puts the edge-less groups last) their loads are issued branch-free,
 // every group's before any group is counted, so the dependent loads of the
 // groups (offsets, neighbour ids, labels) overlap; otherwise, and with the
@@ -194,7 +196,7 @@ __device__ __forceinline__ void mark_vertex(const StepArgs& a, int32_t v, int32_
 // frequency; it reads back the sum for its pair, scores it, and an LDS
 // 64-bit max per owner slot keeps each vertex's best packed score.  Sums and
 // maxima are order-free integers, as in every tier.
-constexpr int kPackNG = 2;
+constexpr int kFoldNG = 2;  // groups per wave in folded (multi-job) launches
 constexpr uint32_t kPackSlots = 128;
 
 // One group's state between its loads and its count.  Vertex lane i owns
@@ -332,24 +334,24 @@ __device__ __forceinline__ uint32_t packed_finish(const StepArgs& a, const Packe
 }
 
 // ngroups: groups to run; groups [0, nge) have edges.
-template <class H, bool W, bool G>
+template <class H, bool W, bool G, int NG>
 __device__ __forceinline__ void packed_body(const StepArgs& a, const int2* __restrict__ groups,
                                             int32_t ngroups, int32_t nge, uint32_t blk) {
     __shared__ PackedLds s_pk[4];
     // wave-uniform first group (readfirstlane keeps the group loads scalar)
     const int32_t g0 =
-        __builtin_amdgcn_readfirstlane((int32_t)((blk * 256u + threadIdx.x) >> 6)) * kPackNG;
+        __builtin_amdgcn_readfirstlane((int32_t)((blk * 256u + threadIdx.x) >> 6)) * NG;
     if (g0 >= ngroups) return;
     PackedLds& t = s_pk[threadIdx.x >> 6];
     uint32_t nch = 0u;
-    if (g0 + kPackNG <= nge && !frontier_on<H>(a)) {
-        PackedGroup s[kPackNG];
+    if (g0 + NG <= nge && !frontier_on<H>(a)) {
+        PackedGroup s[NG];
 #pragma unroll
-        for (int g = 0; g < kPackNG; ++g) packed_load_fast<H, W, G>(a, groups[g0 + g], s[g]);
+        for (int g = 0; g < NG; ++g) packed_load_fast<H, W, G>(a, groups[g0 + g], s[g]);
 #pragma unroll
-        for (int g = 0; g < kPackNG; ++g) nch += packed_finish<H, W, G>(a, s[g], t);
+        for (int g = 0; g < NG; ++g) nch += packed_finish<H, W, G>(a, s[g], t);
     } else {
-        for (int g = 0; g < kPackNG && g0 + g < ngroups; ++g) {
+        for (int g = 0; g < NG && g0 + g < ngroups; ++g) {
             PackedGroup s;
             if (packed_load_safe<H, W, G>(a, groups[g0 + g], s)) nch += packed_finish<H, W, G>(a, s, t);
         }
@@ -361,8 +363,8 @@ __device__ __forceinline__ void packed_body(const StepArgs& a, const int2* __res
 // Wave-tier vertices of degree <= half_max_deg (<= kHalfMaxDeg): a wave
 // counts two at a time, lanes 0-31 one vertex and lanes 32-63 the other, two
 // edges per lane, each half in its own LDS table of kHalfCap slots.  A wave
-// takes kHalfPairs such pairs, their loads issued before any is counted.
-constexpr int kHalfPairs = 2;
+// takes NP such pairs (chosen per launch like the packed tier's NG), their
+// loads issued before any is counted.
 constexpr uint32_t kHalfCap = 2 * kHalfMaxDeg;
 
 // Per-wave LDS of the half tier: one (key << 32 | sum) table per half and
@@ -372,24 +374,24 @@ struct HalfLds {
     unsigned long long best[2];
 };
 
-template <class H, bool W, bool G>
+template <class H, bool W, bool G, int NP>
 __device__ __forceinline__ void half_body(const StepArgs& a, const int32_t* __restrict__ verts,
                                           int32_t nv, uint32_t blk) {
     __shared__ HalfLds s_half[4];
     const int lane = lane_id(), wv = threadIdx.x >> 6, hf = lane >> 5, hl = lane & 31;
-    const int32_t i0 = ((int32_t)blk * 4 + wv) * (2 * kHalfPairs);
+    const int32_t i0 = ((int32_t)blk * 4 + wv) * (2 * NP);
     if (i0 >= nv) return;  // wave-uniform
-    int32_t v[kHalfPairs], b[kHalfPairs], e[kHalfPairs], old[kHalfPairs];
-    bool run[kHalfPairs];
-    uint32_t key[kHalfPairs][2], fr[kHalfPairs][2];
+    int32_t v[NP], b[NP], e[NP], old[NP];
+    bool run[NP];
+    uint32_t key[NP][2], fr[NP][2];
 #pragma unroll
-    for (int p = 0; p < kHalfPairs; ++p) {
+    for (int p = 0; p < NP; ++p) {
         const int32_t idx = i0 + 2 * p + hf;
         v[p] = verts[min(idx, nv - 1)];
         run[p] = idx < nv;
     }
 #pragma unroll
-    for (int p = 0; p < kHalfPairs; ++p) {
+    for (int p = 0; p < NP; ++p) {
         run[p] = run[p] && is_active<H>(a, v[p]) && !is_fixed<H>(a, v[p]);
         b[p] = a.off[v[p]];
         e[p] = a.off[v[p] + 1];
@@ -398,7 +400,7 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int32_t* __re
     // every tier vertex has >= 1 edge, so e-1 is a valid index; a vertex not
     // run loads its first edge only
 #pragma unroll
-    for (int p = 0; p < kHalfPairs; ++p)
+    for (int p = 0; p < NP; ++p)
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             const int32_t j = b[p] + hl + 32 * i;
@@ -409,55 +411,88 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int32_t* __re
     unsigned long long* T = t.slot[hf];
     uint32_t nch = 0u;
 #pragma unroll
-    for (int p = 0; p < kHalfPairs; ++p) {
+    for (int p = 0; p < NP; ++p) {
         if (__ballot(run[p]) == 0ull) continue;  // wave-uniform: both halves idle
-        wave_lds_sync();  // the previous pair's reads are done
-#pragma unroll
-        for (uint32_t i = 0; i < kHalfCap; i += 32) T[i + hl] = 0ull;
-        if (hl == 0) t.best[hf] = 0ull;
-        wave_lds_sync();
-        // per half, the lanes holding the half's first pending label merge
-        // into one insert by the lowest of them (a converged vertex shows one
-        // label on most edges, and same-address LDS atomics serialise); the
-        // merged lanes neither insert nor score (the leader's slot holds the
-        // same count)
-        uint32_t hs[2], sk[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            uint32_t k = key[p][i], f = W ? fr[p][i] : 1u;
-            const unsigned long long rest = __ballot(k != 0u);
-            const uint32_t rlo = (uint32_t)rest, rhi = (uint32_t)(rest >> 32);
-            const int flo = rlo ? (int)__builtin_ctz(rlo) : 0;
-            const int fhi = rhi ? 32 + (int)__builtin_ctz(rhi) : 32;
-            const uint32_t llo = rlo ? (uint32_t)__builtin_amdgcn_readlane((int)k, flo) : 0u;
-            const uint32_t lhi = rhi ? (uint32_t)__builtin_amdgcn_readlane((int)k, fhi) : 0u;
+        // majority shortcut (scores that are frequency sums): when the label
+        // of the half's first edge holds more than half of the half's counted
+        // frequency, no other label can reach its score
+        uint64_t fast = 0ull;
+        if constexpr (count_score_of<H>::value) {
+            const uint32_t k0 = key[p][0], k1 = key[p][1];
+            const uint32_t llo = (uint32_t)__builtin_amdgcn_readlane((int)k0, 0);
+            const uint32_t lhi = (uint32_t)__builtin_amdgcn_readlane((int)k0, 32);
             const uint32_t lead = hf ? lhi : llo;
-            const bool same = k != 0u && k == lead;
-            const unsigned long long sm = __ballot(same);
-            const uint32_t cnt = (uint32_t)__popc(hf ? (uint32_t)(sm >> 32) : (uint32_t)sm);
+            uint32_t c, tot;
             if (W) {
-                // weighted: the half's sum of the merged lanes' frequencies
-                uint32_t x = same ? f : 0u;
+                c = (k0 && k0 == lead ? fr[p][0] : 0u) + (k1 && k1 == lead ? fr[p][1] : 0u);
+                tot = (k0 ? fr[p][0] : 0u) + (k1 ? fr[p][1] : 0u);
 #pragma unroll
-                for (int d = 16; d >= 1; d >>= 1) x += shfl_u32(x, lane ^ d);
-                if (same && cnt >= 2u) f = x;
-            } else if (same && cnt >= 2u) {
-                f = cnt;
+                for (int d = 16; d >= 1; d >>= 1) {
+                    c += shfl_u32(c, lane ^ d);
+                    tot += shfl_u32(tot, lane ^ d);
+                }
+            } else {
+                const unsigned long long m0 = __ballot(k0 != 0u && k0 == lead);
+                const unsigned long long m1 = __ballot(k1 != 0u && k1 == lead);
+                const unsigned long long v0 = __ballot(k0 != 0u), v1 = __ballot(k1 != 0u);
+                const int sh = hf ? 32 : 0;
+                c = (uint32_t)(__popc((uint32_t)(m0 >> sh)) + __popc((uint32_t)(m1 >> sh)));
+                tot = (uint32_t)(__popc((uint32_t)(v0 >> sh)) + __popc((uint32_t)(v1 >> sh)));
             }
-            if (same && cnt >= 2u && lane != (hf ? fhi : flo)) k = 0u;  // merged into the leader
-            sk[i] = k;
-            hs[i] = k ? kc_insert(T, kHalfCap - 1u, k, f) : 0u;
+            if (run[p] && lead != 0u && 2ull * c > (unsigned long long)tot)
+                fast = H::label_score(a, v[p], lead, c);
         }
-        wave_lds_sync();
-        // one score per distinct label: the slot's creator
-        uint64_t pm = 0ull;
+        uint64_t best = fast;
+        if (__ballot(run[p] && fast == 0ull) != 0ull) {  // wave-uniform: some half needs the table
+            wave_lds_sync();  // the previous pair's reads are done
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
-            if (sk[i] && (hs[i] & kClaimed))
-                pm = max_u64(pm, H::label_score(a, v[p], sk[i], (uint32_t)T[hs[i] & ~kClaimed]));
-        if (pm) atomicMax(&t.best[hf], (unsigned long long)pm);
-        wave_lds_sync();
-        const uint64_t best = t.best[hf];
+            for (uint32_t i = 0; i < kHalfCap; i += 32) T[i + hl] = 0ull;
+            if (hl == 0) t.best[hf] = 0ull;
+            wave_lds_sync();
+            // per half, the lanes holding the half's first pending label
+            // merge into one insert by the lowest of them (a converged vertex
+            // shows one label on most edges, and same-address LDS atomics
+            // serialise); the merged lanes neither insert nor score (the
+            // leader's slot holds the same count).  A half decided by the
+            // shortcut inserts nothing.
+            uint32_t hs[2], sk[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                uint32_t k = fast ? 0u : key[p][i], f = W ? fr[p][i] : 1u;
+                const unsigned long long rest = __ballot(k != 0u);
+                const uint32_t rlo = (uint32_t)rest, rhi = (uint32_t)(rest >> 32);
+                const int flo = rlo ? (int)__builtin_ctz(rlo) : 0;
+                const int fhi = rhi ? 32 + (int)__builtin_ctz(rhi) : 32;
+                const uint32_t llo = rlo ? (uint32_t)__builtin_amdgcn_readlane((int)k, flo) : 0u;
+                const uint32_t lhi = rhi ? (uint32_t)__builtin_amdgcn_readlane((int)k, fhi) : 0u;
+                const uint32_t lead = hf ? lhi : llo;
+                const bool same = k != 0u && k == lead;
+                const unsigned long long sm = __ballot(same);
+                const uint32_t cnt = (uint32_t)__popc(hf ? (uint32_t)(sm >> 32) : (uint32_t)sm);
+                if (W) {
+                    // weighted: the half's sum of the merged lanes' frequencies
+                    uint32_t x = same ? f : 0u;
+#pragma unroll
+                    for (int d = 16; d >= 1; d >>= 1) x += shfl_u32(x, lane ^ d);
+                    if (same && cnt >= 2u) f = x;
+                } else if (same && cnt >= 2u) {
+                    f = cnt;
+                }
+                if (same && cnt >= 2u && lane != (hf ? fhi : flo)) k = 0u;  // merged into the leader
+                sk[i] = k;
+                hs[i] = k ? kc_insert(T, kHalfCap - 1u, k, f) : 0u;
+            }
+            wave_lds_sync();
+            // one score per distinct label: the slot's creator
+            uint64_t pm = 0ull;
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                if (sk[i] && (hs[i] & kClaimed))
+                    pm = max_u64(pm, H::label_score(a, v[p], sk[i], (uint32_t)T[hs[i] & ~kClaimed]));
+            if (pm) atomicMax(&t.best[hf], (unsigned long long)pm);
+            wave_lds_sync();
+            if (!fast) best = t.best[hf];
+        }
         bool changed = false;
         if (hl == 0 && run[p]) {
             const int32_t nl = best ? packed_label(best) : old[p];
@@ -512,20 +547,48 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int32_t* __re
 #pragma unroll
     for (int k = 0; k < kWaveNV; ++k) {
         if (!run[k]) continue;  // wave-uniform
-        const uint32_t cap = min(max(next_pow2(2u * (uint32_t)(e[k] - b[k])), 64u), (uint32_t)kWaveCap);
-        const uint32_t mask = cap - 1u;
-        for (uint32_t i = lane; i < cap; i += 64) T[i] = 0ull;
-        wave_lds_sync();
-#pragma unroll
-        for (int i = 0; i < 4; ++i) wave_insert<W>(T, mask, key[k][i], fr[k][i]);
-        wave_lds_sync();
         uint64_t best = 0ull;
-        for (uint32_t i = lane; i < cap; i += 64) {
-            const unsigned long long x = T[i];
-            if (x) best = max_u64(best, H::label_score(a, v[k], (uint32_t)(x >> 32), (uint32_t)x));
+        bool decided = false;  // wave-uniform
+        if constexpr (count_score_of<H>::value) {
+            // majority shortcut: the label of lane 0's first edge wins outright
+            // when it holds more than half of the counted frequency
+            const uint32_t lead = (uint32_t)__builtin_amdgcn_readlane((int)key[k][0], 0);
+            uint32_t c = 0u, tot = 0u;
+            if (W) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    c += key[k][i] && key[k][i] == lead ? fr[k][i] : 0u;
+                    tot += key[k][i] ? fr[k][i] : 0u;
+                }
+                c = wave_sum_u32(c);
+                tot = wave_sum_u32(tot);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    c += (uint32_t)__popcll(__ballot(key[k][i] != 0u && key[k][i] == lead));
+                    tot += (uint32_t)__popcll(__ballot(key[k][i] != 0u));
+                }
+            }
+            if (lead != 0u && 2ull * c > (unsigned long long)tot) {
+                best = H::label_score(a, v[k], lead, c);
+                decided = true;
+            }
         }
-        best = wave_max_u64(best);
-        wave_lds_sync();  // the next vertex clears what this scan read
+        if (!decided) {
+            const uint32_t cap = min(max(next_pow2(2u * (uint32_t)(e[k] - b[k])), 64u), (uint32_t)kWaveCap);
+            const uint32_t mask = cap - 1u;
+            for (uint32_t i = lane; i < cap; i += 64) T[i] = 0ull;
+            wave_lds_sync();
+#pragma unroll
+            for (int i = 0; i < 4; ++i) wave_insert<W>(T, mask, key[k][i], fr[k][i]);
+            wave_lds_sync();
+            for (uint32_t i = lane; i < cap; i += 64) {
+                const unsigned long long x = T[i];
+                if (x) best = max_u64(best, H::label_score(a, v[k], (uint32_t)(x >> 32), (uint32_t)x));
+            }
+            best = wave_max_u64(best);
+            wave_lds_sync();  // the next vertex clears what this scan read
+        }
         bool changed = false;
         if (lane == 0) {
             const int32_t nl = best ? packed_label(best) : old[k];
@@ -919,15 +982,15 @@ __device__ __forceinline__ void hub_mark_body(const StepArgs& a, const HubArgs& 
 
 // ------------------------------------------------------------- wrappers ----
 // Single job: workgroup index = blockIdx.x.
-template <class H, bool W, bool G>
+template <class H, bool W, bool G, int NG>
 __global__ __launch_bounds__(256) void k_packed(StepArgs a, const int2* __restrict__ groups,
                                                 int32_t ngroups, int32_t nge) {
-    packed_body<H, W, G>(a, groups, ngroups, nge, blockIdx.x);
+    packed_body<H, W, G, NG>(a, groups, ngroups, nge, blockIdx.x);
 }
-template <class H, bool W, bool G>
+template <class H, bool W, bool G, int NP>
 __global__ __launch_bounds__(256) void k_half(StepArgs a, const int32_t* __restrict__ verts,
                                               int32_t nv) {
-    half_body<H, W, G>(a, verts, nv, blockIdx.x);
+    half_body<H, W, G, NP>(a, verts, nv, blockIdx.x);
 }
 template <class H, bool W, bool G>
 __global__ __launch_bounds__(256) void k_wave(StepArgs a, const int32_t* __restrict__ verts,
@@ -983,9 +1046,9 @@ __global__ __launch_bounds__(NT) void k_multi(const JobDesc* __restrict__ jobs,
     const uint32_t blk = blockIdx.x - (uint32_t)cum[j];
     const JobDesc& jd = jobs[j];
     if constexpr (L == kLaunchPacked) {
-        packed_body<H, W, G>(jd.a, jd.groups, jd.ngroups, jd.ngroups_e, blk);
+        packed_body<H, W, G, kFoldNG>(jd.a, jd.groups, jd.ngroups, jd.ngroups_e, blk);
     } else if constexpr (L == kLaunchHalf) {
-        half_body<H, W, G>(jd.a, jd.half, jd.nhalf, blk);
+        half_body<H, W, G, kFoldNG>(jd.a, jd.half, jd.nhalf, blk);
     } else if constexpr (L == kLaunchWave) {
         wave_body<H, W, G>(jd.a, jd.wave, jd.nwave, blk);
     } else if constexpr (L >= kLaunchBlock0 && L < kLaunchBlock0 + kMaxBlockClasses) {
@@ -1073,8 +1136,8 @@ void job_blocks(const Plan& p, int32_t* blocks) {
     for (int L = 0; L < kLaunches; ++L) blocks[L] = 0;
     // folded jobs run the layered hooks, which skip edge-less groups
     static_assert(LayeredHooks::kSkipIsolated, "folded launches run groups with edges only");
-    blocks[kLaunchPacked] = (int32_t)((p.groups_with_edges + 4 * kPackNG - 1) / (4 * kPackNG));
-    blocks[kLaunchHalf] = (int32_t)((p.half.size() + 8 * kHalfPairs - 1) / (8 * kHalfPairs));
+    blocks[kLaunchPacked] = (int32_t)((p.groups_with_edges + 4 * kFoldNG - 1) / (4 * kFoldNG));
+    blocks[kLaunchHalf] = (int32_t)((p.half.size() + 8 * kFoldNG - 1) / (8 * kFoldNG));
     blocks[kLaunchWave] = (int32_t)((p.wave.size() + 4 * kWaveNV - 1) / (4 * kWaveNV));
     for (int c = 0; c < p.nclasses; ++c) blocks[kLaunchBlock0 + c] = (int32_t)p.block[c].size();
     blocks[kLaunchHubAgg] = (int32_t)p.hub_chunks.size();
@@ -1256,6 +1319,15 @@ __global__ __launch_bounds__(kChangeSlots) void k_end_step(unsigned long long* a
 }
 
 // ---------------------------------------------------------------- launch ----
+// Units per wave for a launch of `units` work units: 4 when that still
+// leaves >= kFillWaves waves, else 2 (>= kFillWaves / 4 waves), else 1.
+constexpr int64_t kFillWaves = 16384;
+static inline int units_per_wave(int64_t units) {
+    if (units / 4 >= kFillWaves) return 4;
+    if (units / 2 >= kFillWaves / 4) return 2;
+    return 1;
+}
+
 static inline unsigned grid_for(int64_t work, int per_block) {
     int64_t g = (work + per_block - 1) / per_block;
     return (unsigned)std::max<int64_t>(1, g);
@@ -1271,15 +1343,39 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
     mark(GLP_TIER_PACKED, 0);
     const int32_t ng = (int32_t)(H::kSkipIsolated ? p.groups_with_edges : (int64_t)p.groups.size());
     if (ng > 0) {
-        hipLaunchKernelGGL((k_packed<H, W, G>), dim3(grid_for(ng, 4 * kPackNG)), dim3(256), 0, s, a,
-                           g.d_groups, ng, (int32_t)p.groups_with_edges);
+        const int32_t nge = (int32_t)p.groups_with_edges;
+        switch (units_per_wave(ng)) {
+            case 4:
+                hipLaunchKernelGGL((k_packed<H, W, G, 4>), dim3(grid_for(ng, 16)), dim3(256), 0, s, a,
+                                   g.d_groups, ng, nge);
+                break;
+            case 2:
+                hipLaunchKernelGGL((k_packed<H, W, G, 2>), dim3(grid_for(ng, 8)), dim3(256), 0, s, a,
+                                   g.d_groups, ng, nge);
+                break;
+            default:
+                hipLaunchKernelGGL((k_packed<H, W, G, 1>), dim3(grid_for(ng, 4)), dim3(256), 0, s, a,
+                                   g.d_groups, ng, nge);
+        }
     }
     mark(GLP_TIER_PACKED, 1);
     mark(GLP_TIER_WAVE, 0);
     if (!p.half.empty()) {
         const int32_t nh = (int32_t)p.half.size();
-        hipLaunchKernelGGL((k_half<H, W, G>), dim3(grid_for(nh, 8 * kHalfPairs)), dim3(256), 0, s, a,
-                           g.d_half, nh);
+        // a pair of vertices per wave is one unit
+        switch (units_per_wave((nh + 1) / 2)) {
+            case 4:
+                hipLaunchKernelGGL((k_half<H, W, G, 4>), dim3(grid_for(nh, 32)), dim3(256), 0, s, a,
+                                   g.d_half, nh);
+                break;
+            case 2:
+                hipLaunchKernelGGL((k_half<H, W, G, 2>), dim3(grid_for(nh, 16)), dim3(256), 0, s, a,
+                                   g.d_half, nh);
+                break;
+            default:
+                hipLaunchKernelGGL((k_half<H, W, G, 1>), dim3(grid_for(nh, 8)), dim3(256), 0, s, a,
+                                   g.d_half, nh);
+        }
     }
     if (!p.wave.empty()) {
         const int32_t nw = (int32_t)p.wave.size();
