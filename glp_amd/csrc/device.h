@@ -105,10 +105,11 @@ struct HubScratch {
 };
 
 // Buckets of a hub of degree d (host and device agree): enough that a bucket
-// expects at most part_keys distinct labels, at most kMaxBuckets.
+// expects at most 3/4 part_keys distinct labels, at most kMaxBuckets.
 constexpr int kMaxBuckets = 4096;
 __host__ __device__ __forceinline__ uint32_t hub_buckets(int32_t d, int32_t part_keys) {
-    const uint32_t p = (uint32_t)(((int64_t)d + part_keys - 1) / part_keys);
+    const int64_t per = 3 * (int64_t)part_keys;  // buckets = ceil(4 d / (3 part_keys))
+    const uint32_t p = (uint32_t)((4 * (int64_t)d + per - 1) / per);
     return p < 1u ? 1u : (p > (uint32_t)kMaxBuckets ? (uint32_t)kMaxBuckets : p);
 }
 

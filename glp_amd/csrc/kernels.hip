@@ -673,8 +673,9 @@ __device__ __forceinline__ void block_body(const StepArgs& a, const int32_t* __r
 //                then the rest, halving as needed (fmix32 is a bijection, so a
 //                range no wider than the table cannot overflow)
 //  k_hub_final   one thread per hub: commits the winner
-// P = ceil(degree / part_keys), at most kMaxBuckets, so a bucket expects at
-// most part_keys distinct labels even when every neighbour shows a different
+// P = ceil(4 degree / (3 part_keys)), at most kMaxBuckets, so a bucket expects
+// at most 3/4 part_keys distinct labels (its table of 2 part_keys slots takes
+// part_keys before halving) even when every neighbour shows a different
 // label; once labels converge most buckets are empty and exit at once.
 constexpr int kHubThreads = 256;
 
@@ -872,7 +873,7 @@ __device__ __forceinline__ void hub_part_count(const StepArgs& a, const HubArgs&
     extern __shared__ __align__(16) uint32_t s_dyn[];
     const unsigned long long* pr = h.hs.sorted + h.base[j.hub] + j.start;
     const uint32_t n = j.stop - j.start;
-    const uint32_t cap_max = next_pow2(4u * (uint32_t)h.part_keys);
+    const uint32_t cap_max = next_pow2(2u * (uint32_t)h.part_keys);
     // a bucket of n <= cap_max/2 pairs gets a table of >= 2n slots: it holds
     // at most n distinct labels, so it cannot overflow, and converged buckets
     // (a few pairs) scan a few slots instead of the whole table
@@ -938,7 +939,7 @@ __device__ __forceinline__ void hub_part_batch_body(const StepArgs& a, const Hub
     __syncthreads();
     const uint32_t n = s_n;
     if (n == 0u) return;  // block-uniform
-    const uint32_t words = 2u * next_pow2(4u * (uint32_t)h.part_keys);
+    const uint32_t words = 2u * next_pow2(2u * (uint32_t)h.part_keys);
     for (uint32_t i = threadIdx.x; i < words; i += kHubThreads) s_dyn[i] = 0u;
     // (the first count's barrier orders this clear before its inserts)
     for (uint32_t i = 0; i < n; ++i) hub_part_count<H>(a, h, s_jobs[i], s_red, &s_distinct, &s_over);
@@ -1114,7 +1115,7 @@ static int launch_multi_t(const JobDesc* jobs, const int32_t* cum, int32_t nj, c
                            dim3(grid[kLaunchHubScatter]), dim3(kHubThreads), 0, s, jobs,
                            row(kLaunchHubScatter), nj, 0u);
     if (grid[kLaunchHubPart]) {
-        const size_t lds = (size_t)next_pow2(4u * (uint32_t)t.part_keys) * 2 * sizeof(uint32_t);
+        const size_t lds = (size_t)next_pow2(2u * (uint32_t)t.part_keys) * 2 * sizeof(uint32_t);
         hipLaunchKernelGGL((k_multi<H, W, false, kLaunchHubPart, kHubThreads>), dim3(grid[kLaunchHubPart]),
                            dim3(kHubThreads), lds, s, jobs, row(kLaunchHubPart), nj, 0u);
     }
@@ -1418,7 +1419,7 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
         h.hs = hs;
         const unsigned nc = (unsigned)p.hub_chunks.size();
         const unsigned np = (unsigned)p.hub_parts.size();
-        const size_t lds_part = (size_t)next_pow2(4u * (uint32_t)h.part_keys) * 2 * sizeof(uint32_t);
+        const size_t lds_part = (size_t)next_pow2(2u * (uint32_t)h.part_keys) * 2 * sizeof(uint32_t);
         hipLaunchKernelGGL((k_hub_agg<H, W, G>), dim3(nc), dim3(kHubThreads), hub_agg_lds(h.chunk), s, a,
                            h);
         mark(GLP_TIER_HUB, 1);

@@ -81,13 +81,15 @@ static int check_tuning(const Tuning& t) {
     // the largest LDS table (2*deg slots of 8 bytes) must fit a CU's 160 KB
     if (t.block_max_deg < t.wave_max_deg || t.block_max_deg > 8192)
         return set_error(GLP_E_ARG, "block_max_deg must be in [wave_max_deg, 8192]");
-    // LDS tables: next_pow2(2*hub_chunk) and next_pow2(4*part_keys) slots of 8 bytes
+    // LDS tables: next_pow2(2*hub_chunk) and next_pow2(2*part_keys) slots of 8 bytes
     if (t.hub_chunk < 256 || t.hub_chunk > 8192 || (t.hub_chunk & 255))
         return set_error(GLP_E_ARG, "hub_chunk must be a multiple of 256 in [256, 8192]");
     if (t.part_keys < 256 || t.part_keys > 4096)
         return set_error(GLP_E_ARG, "part_keys must be in [256, 4096]");
-    if (t.part_limit < 0 || t.part_limit > 2 * t.part_keys)
-        return set_error(GLP_E_ARG, "part_limit must be in [0, 2*part_keys]");
+    // a pass stops after `limit` distinct labels (+ at most a workgroup's
+    // worth in flight): limit <= part_keys keeps that inside the table
+    if (t.part_limit < 0 || t.part_limit > t.part_keys)
+        return set_error(GLP_E_ARG, "part_limit must be in [0, part_keys]");
     if (t.block_threads != 256 && t.block_threads != 512 && t.block_threads != 1024)
         return set_error(GLP_E_ARG, "block_threads must be 256, 512 or 1024");
     return GLP_OK;
