@@ -189,6 +189,22 @@ def bench_served(args, G):
     }
 
 
+RR_TABLE = 1 << 25    # int32 entries: 128 MiB, the size of RMAT-25's label array
+RR_READS = 1 << 28
+
+
+def bench_random_read(args, G):
+    """Calibration of the label gather's access pattern: random 4-byte reads
+    from a 128 MiB and a 1 GiB table (library kernel k_calib_read)."""
+    out = {"metric": "random 4-byte read calibration", "unit": "G reads/s", "reads_per_launch": RR_READS}
+    for entries in (RR_TABLE, 1 << 28):
+        ms = G.calib_random_read(entries, RR_READS, 5)
+        out["table_%dMiB" % (entries * 4 >> 20)] = {"ms": round(ms, 4),
+                                                     "G_reads_per_s": round(RR_READS / (ms / 1e3) / 1e9, 2)}
+    out["value"] = out["table_%dMiB" % (RR_TABLE * 4 >> 20)]["G_reads_per_s"]
+    return out
+
+
 def bench_copy(args):
     """Calibration for rocprof byte counters: device-to-device copies of a
     known size (the runtime's copy kernel), so FETCH_SIZE / WRITE_SIZE per
@@ -233,7 +249,8 @@ def main():
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="tuning override (pack_max_deg, wave_max_deg, block_max_deg, hub_chunk, "
                          "part_keys, part_limit, block_threads); repeatable")
-    ap.add_argument("--workload", choices=["rmat", "batched", "served", "copy"], default="rmat")
+    ap.add_argument("--workload", choices=["rmat", "batched", "served", "copy", "random-read"],
+                    default="rmat")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--graphs", type=int, default=1024)
     ap.add_argument("--no-autotune", action="store_true")
@@ -264,6 +281,9 @@ def main():
         return
     if args.workload == "copy":
         print(json.dumps(bench_copy(args)), flush=True)
+        return
+    if args.workload == "random-read":
+        print(json.dumps(bench_random_read(args, G)), flush=True)
         return
     threads = min(len(os.sched_getaffinity(0)), 16)
     log("generating RMAT scale %d ef %d seed %d (%d threads)" % (args.scale, args.edge_factor,
@@ -318,11 +338,20 @@ def main():
                     "traffic": None if traffic is None else int(traffic),
                     "kernel": kernel_names[dom], "kernel_ms": round(tiers[dom]["ms"], 4),
                     "algorithmic_bytes": int(b)}
+        # the ceiling of the label gather's access pattern on this device:
+        # random 4-byte reads of a label-array-sized table (each one a 128-byte
+        # line fetch when it misses L2)
+        rr_ms = G.calib_random_read(RR_TABLE, RR_READS, 5)
+        rr = RR_READS / (rr_ms / 1e3) / 1e9
+        roofline["random_read_ceiling"] = {"G_reads_per_s": round(rr, 2),
+                                           "GBs_in_128B_lines": round(rr * 128, 1),
+                                           "table_MiB": RR_TABLE * 4 >> 20}
         if traffic is not None:
             # measured HBM bytes (rocprof, calibrated) over this run's duration
             tg = traffic / (tiers[dom]["ms"] / 1e3) / 1e9
             roofline["traffic_GBs"] = round(tg, 1)
             roofline["traffic_frac"] = round(tg / HBM_PEAK_GBS, 4)
+            roofline["traffic_frac_of_random_read_ceiling"] = round(tg / (rr * 128), 4)
     whole = tier_bytes(m, n)
     step_roofline = {"algorithmic_bytes": int(whole),
                      "achieved_GBs": round(whole / (ms_per_step / 1e3) / 1e9, 1),

@@ -97,6 +97,8 @@ _sig("glp_graph_degree_sort", ctypes.c_int, _vp, _pp, _c_i32p)
 _sig("glp_tool_edgelist2bin", ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p)
 _sig("glp_tool_normalize", ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p)
 _sig("glp_tool_direct_edgelist2bin", ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32)
+_sig("glp_calib_random_read", ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
+     ctypes.POINTER(ctypes.c_double))
 _sig("glp_gen_rmat", ctypes.c_int, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
      ctypes.c_double, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _pp)
 _sig("glp_gen_planted", ctypes.c_int, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
@@ -383,6 +385,15 @@ def direct_edgelist2bin(bin_path, out_path, id_cutoff=DIRECT_ID_CUTOFF):
     keeps every edge."""
     _check(_lib.glp_tool_direct_edgelist2bin(os.fsencode(bin_path), os.fsencode(out_path),
                                              int(id_cutoff)))
+
+
+def calib_random_read(table_entries=1 << 25, reads=1 << 28, repeats=5):
+    """Fastest time (ms) of one launch of `reads` random 4-byte reads from a
+    table of table_entries int32 (a power of two): the device's ceiling for
+    the label gather's access pattern."""
+    ms = ctypes.c_double(0.0)
+    _check(_lib.glp_calib_random_read(int(table_entries), int(reads), int(repeats), ctypes.byref(ms)))
+    return ms.value
 
 
 class Session:

@@ -61,6 +61,13 @@ struct Plan {
 int build_plan(const HostCSR& g, const Tuning& t, Plan& p);
 
 // HBM copy of a graph plus its plan (shared by all sessions on the graph).
+// For each vertex u, the CSR positions j with nbr[j] = u: off[n+1], pos[m]
+// (built on the device when a frontier gather first needs it).
+struct InverseIndex {
+    int32_t* off = nullptr;
+    int32_t* pos = nullptr;
+};
+
 struct DeviceGraph {
     int32_t n = 0;
     int64_t m = 0;
@@ -78,6 +85,7 @@ struct DeviceGraph {
     int64_t* d_hub_base = nullptr;
     int2* d_hub_parts = nullptr;
     int64_t* d_hub_hist_base = nullptr;
+    InverseIndex inv;
     double upload_ms = 0, plan_ms = 0;
     size_t bytes = 0;
     bool owns_csr = true;  // false: off/nbr/w borrowed from another DeviceGraph
@@ -158,6 +166,24 @@ int launch_step(Method method, bool weighted, bool gathered, const DeviceGraph& 
                 const HubScratch& hs, const StepArgs& a, hipStream_t s, hipEvent_t* ev);
 int launch_pick(Method method, const StepArgs& a, int32_t* shown, hipStream_t s);
 int launch_gather(const StepArgs& a, int64_t m, int32_t* adj, hipStream_t s);
+// Builds inv (arrays allocated by the caller) from nbr; cursor is n+1 int32
+// of scratch.  Synchronises s.
+int inverse_index_build(const int32_t* nbr, int32_t n, int64_t m, int32_t* cursor, InverseIndex& inv,
+                        hipStream_t s);
+// Most in-edges of changed sources for which a frontier gather pushes
+// (frontier control word 7); above it the gather rewrites every edge.
+int32_t push_edge_limit(int64_t m);
+// Random-read calibration: `reads` 4-byte reads of T (entries a power of
+// two) at uniform hashed indices; out holds out_entries int32 sums.
+int launch_calib_read(const int32_t* T, int64_t entries, int64_t reads, uint32_t seed, int32_t* out,
+                      int64_t out_entries, hipStream_t s);
+// Bytes of the push-entry list a frontier gather over (n, m) needs.
+size_t push_entry_bytes(int32_t n, int64_t m);
+// The gather of a frontier step (a.chg_r, a.fctl set): pushes the labels of
+// the sources that changed in the previous step through the inverse index,
+// or rewrites every edge after a step with many changes.
+int launch_frontier_gather(const StepArgs& a, int64_t m, int32_t* adj, const InverseIndex& inv,
+                           void* push, hipStream_t s);
 int launch_fill_i32(int32_t* p, int32_t value, int64_t count, hipStream_t s);
 int launch_iota(int32_t* p, int64_t count, hipStream_t s);
 // V[l] += #{v < n : L[v] = l} (layered LP's volume of the written labels;

@@ -51,8 +51,8 @@ struct StepArgs {
     const int32_t* fctl;
     // frontier gather (separate gather with the frontier on; null otherwise):
     // bit v of chg_w is set when v's label changes in this step; chg_r holds
-    // the previous step's bits, and when fctl[1] = 1 (that step changed few
-    // labels) the gather rewrites only the edges whose source bit is set
+    // the previous step's bits, whose sources' labels the gather pushes to
+    // their in-edges when they have few
     const uint32_t* chg_r;
     uint32_t* chg_w;
     // seeded LP: seeds[v] != 0 marks a seed (its label stays v)

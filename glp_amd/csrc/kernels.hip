@@ -1184,35 +1184,257 @@ __global__ __launch_bounds__(256) void k_gather(const int32_t* __restrict__ nbr,
     gather_all(nbr, L, adj, m);
 }
 
-// Frontier gather (header/kernel.cuh:21-35's if_update gather, with a bitmap
-// of the sources that changed in the previous step).  adj already holds the
-// labels of two steps back, so rewriting the edges whose source changed makes
-// it equal L[nbr]; after a step that changed many labels (fctl[1] = 0, also
-// the first step) every edge is rewritten.
+// ---------------------------------------------------- frontier gather ----
+// The frontier form of the separate gather (header/kernel.cuh:21-78's
+// if_update gathers) pushes labels from sources instead of pulling them per
+// edge.  An inverse index lists, for each vertex u, the CSR positions j with
+// nbr[j] = u (the reference gets the same lists by sorting (nbr[j], j) pairs
+// with thrust::sort_by_key, prop/layeredLprop.cuh:228-237, and reads them in
+// gather_labels_inverse, header/kernel.cuh:64-78; a counting pass builds
+// them here in O(m)).  adj holds the labels of two steps back, so writing
+// L[u] to the positions of every u that changed in the previous step makes
+// it equal L[nbr]: the work is the changed sources' in-edges, not m.
+//
+// Frontier control words used here (the rest of fctl is the tiers'):
+//   fctl[3] = 1 once adj has been gathered (set by k_end_step, 0 after a
+//             reset), fctl[4] = push entries listed for this step,
+//   fctl[5] = in-edges of the sources that changed in the previous step,
+//   fctl[6] = 1 when that step changed at most fctl[7] labels (a changed
+//             vertex has an in-edge, so more changes than that rule the push
+//             out before its sources are counted),
+//   fctl[7] = m / kPushRatio, the most in-edges a push takes (set at reset).
+// A push writes each in-edge's label to a random position; measured on
+// RMAT-25 that costs about five times a streamed edge of the full gather, so
+// the push is taken only below m / 8 in-edges.
+constexpr int32_t kPushChunk = 256;   // in-edges per push entry at most
+constexpr int64_t kPushRatio = 8;
+
+__global__ __launch_bounds__(256) void k_inv_count(const int32_t* __restrict__ nbr, int64_t m,
+                                                   int32_t* __restrict__ cnt) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride)
+        atomicAdd(&cnt[nbr[j]], 1);
+}
+// cursor[u] starts at inv_off[u]; each edge claims the next slot of its
+// source's list (the order inside a list does not matter).
+__global__ __launch_bounds__(256) void k_inv_fill(const int32_t* __restrict__ nbr, int64_t m,
+                                                  int32_t* __restrict__ cursor,
+                                                  int32_t* __restrict__ inv_pos) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride)
+        inv_pos[atomicAdd(&cursor[nbr[j]], 1)] = (int32_t)j;
+}
+
+// Exclusive scan of int32 counts in place (sums < 2^31): block sums, a scan
+// of the sums in one workgroup, then each block rescans its run.
+constexpr int kScanPer = 16;                       // elements per thread
+constexpr int kScanBlock = 256 * kScanPer;         // elements per workgroup
+__device__ __forceinline__ int32_t wave_incl_sum(int32_t x) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int32_t y = shfl_i32(x, lane >= d ? lane - d : lane);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+// Exclusive prefix of x over the workgroup (256 threads); *total = the sum.
+__device__ __forceinline__ int32_t block_excl_sum(int32_t x, int32_t* total) {
+    __shared__ int32_t s_w[4];
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const int32_t incl = wave_incl_sum(x);
+    if (lane == 63) s_w[wv] = incl;
+    __syncthreads();
+    int32_t base = 0, all = 0;
+    for (int i = 0; i < 4; ++i) {
+        if (i < wv) base += s_w[i];
+        all += s_w[i];
+    }
+    __syncthreads();
+    *total = all;
+    return base + incl - x;
+}
+__global__ __launch_bounds__(256) void k_scan_sums(const int32_t* __restrict__ x, int64_t count,
+                                                   int32_t* __restrict__ sums) {
+    const int64_t b0 = (int64_t)blockIdx.x * kScanBlock + (int64_t)threadIdx.x * kScanPer;
+    int32_t s = 0;
+    for (int i = 0; i < kScanPer; ++i)
+        if (b0 + i < count) s += x[b0 + i];
+    int32_t total;
+    (void)block_excl_sum(s, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+__global__ __launch_bounds__(256) void k_scan_top(int32_t* __restrict__ sums, int64_t nsums) {
+    int32_t carry = 0;
+    for (int64_t b = 0; b < nsums; b += 256) {
+        const int64_t i = b + threadIdx.x;
+        const int32_t x = i < nsums ? sums[i] : 0;
+        int32_t total;
+        const int32_t ex = block_excl_sum(x, &total);
+        if (i < nsums) sums[i] = carry + ex;
+        carry += total;
+    }
+}
+__global__ __launch_bounds__(256) void k_scan_apply(int32_t* __restrict__ x, int64_t count,
+                                                    const int32_t* __restrict__ sums) {
+    const int64_t b0 = (int64_t)blockIdx.x * kScanBlock + (int64_t)threadIdx.x * kScanPer;
+    int32_t v[kScanPer];
+    int32_t s = 0;
+    for (int i = 0; i < kScanPer; ++i) {
+        v[i] = b0 + i < count ? x[b0 + i] : 0;
+        s += v[i];
+    }
+    int32_t total;
+    int32_t run = sums[blockIdx.x] + block_excl_sum(s, &total);
+    for (int i = 0; i < kScanPer; ++i) {
+        if (b0 + i < count) x[b0 + i] = run;
+        run += v[i];
+    }
+}
+
+// One push entry: label to write, first in-edge position, in-edge count.
+struct PushEntry {
+    int32_t label, begin, count, pad;
+};
+
+// Push entries of the sources whose bit is set in chg (the previous step's
+// changes), one per kPushChunk in-edges, in vertex order: each thread takes
+// one bitmap word; k_push_count sums every workgroup's entries and in-edges,
+// k_push_scan turns the sums into offsets and totals (fctl[4], fctl[5]), and
+// k_push_write lists the entries when the gather is going to push them.
+// No atomics: the totals of a step with many changes come at the cost of one
+// read of the bitmap.
+__device__ __forceinline__ void push_word_counts(const uint32_t* __restrict__ chg, int64_t w,
+                                                 int64_t nwords, const int32_t* __restrict__ inv_off,
+                                                 int32_t* cnt, int32_t* edges) {
+    const uint32_t bits = w < nwords ? chg[w] : 0u;
+    int32_t c = 0, e = 0;
+    if (__builtin_popcount(bits) >= 8) {
+        // many changed: read the word's 33 offsets at once (inv_off + 32w is
+        // 128-byte aligned, and inv_off has n + 1 >= 32w + 33 entries when
+        // bit 31 can be set; the last word reads up to its highest bit + 1)
+        const int32_t* o = inv_off + w * 32;
+        const int hi = 32 - __builtin_clz(bits);  // highest set bit + 1
+        int32_t prev = o[0];
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            if (i < hi) {
+                const int32_t next = o[i + 1];
+                if ((bits >> i) & 1u) {
+                    const int32_t d = next - prev;
+                    c += (d + kPushChunk - 1) / kPushChunk;
+                    e += d;
+                }
+                prev = next;
+            }
+        }
+    } else {
+        for (uint32_t b = bits; b; b &= b - 1u) {
+            const int32_t u = (int32_t)(w * 32 + __builtin_ctz(b));
+            const int32_t d = inv_off[u + 1] - inv_off[u];
+            c += (d + kPushChunk - 1) / kPushChunk;
+            e += d;
+        }
+    }
+    *cnt = c;
+    *edges = e;
+}
+__device__ __forceinline__ bool push_possible(const int32_t* fctl) {
+    return fctl[3] != 0 && fctl[6] != 0;
+}
+__global__ __launch_bounds__(256) void k_push_count(const uint32_t* __restrict__ chg, int64_t nwords,
+                                                    const int32_t* __restrict__ inv_off,
+                                                    int2* __restrict__ psum,
+                                                    const int32_t* __restrict__ fctl) {
+    if (!push_possible(fctl)) return;  // the gather rewrites every edge this step
+    int32_t c, e, tc, te;
+    push_word_counts(chg, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, nwords, inv_off, &c, &e);
+    (void)block_excl_sum(c, &tc);
+    (void)block_excl_sum(e, &te);
+    if (threadIdx.x == 0) psum[blockIdx.x] = make_int2(tc, te);
+}
+__global__ __launch_bounds__(256) void k_push_scan(int2* __restrict__ psum, int64_t nblocks,
+                                                   int32_t* __restrict__ fctl) {
+    if (!push_possible(fctl)) return;
+    int32_t carry = 0, edges = 0;
+    for (int64_t b = 0; b < nblocks; b += 256) {
+        const int64_t i = b + threadIdx.x;
+        const int2 x = i < nblocks ? psum[i] : make_int2(0, 0);
+        int32_t tc, te;
+        const int32_t ex = block_excl_sum(x.x, &tc);
+        (void)block_excl_sum(x.y, &te);
+        if (i < nblocks) psum[i].x = carry + ex;
+        carry += tc;
+        edges += te;
+    }
+    if (threadIdx.x == 0) {
+        fctl[4] = carry;
+        fctl[5] = edges;
+    }
+}
+__device__ __forceinline__ bool push_taken(const int32_t* fctl) {
+    return push_possible(fctl) && fctl[5] <= fctl[7];
+}
+__global__ __launch_bounds__(256) void k_push_write(const uint32_t* __restrict__ chg, int64_t nwords,
+                                                    const int32_t* __restrict__ inv_off,
+                                                    const int32_t* __restrict__ L,
+                                                    const int2* __restrict__ psum,
+                                                    PushEntry* __restrict__ ent,
+                                                    const int32_t* __restrict__ fctl) {
+    if (!push_taken(fctl)) return;
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t c, e, tc;
+    push_word_counts(chg, w, nwords, inv_off, &c, &e);
+    int32_t k = psum[blockIdx.x].x + block_excl_sum(c, &tc);
+    if (c == 0) return;
+    for (uint32_t b = chg[w]; b; b &= b - 1u) {
+        const int32_t u = (int32_t)(w * 32 + __builtin_ctz(b));
+        const int32_t end = inv_off[u + 1];
+        const int32_t lab = L[u];
+        for (int32_t p = inv_off[u]; p < end; p += kPushChunk)
+            ent[k++] = PushEntry{lab, p, min(kPushChunk, end - p), 0};
+    }
+}
+
+// The gather of a frontier step: the first step, and any step whose changed
+// sources have more than m / kPushRatio in-edges, rewrite every edge
+// (gather_all); otherwise each wave takes 64 push entries at a time and
+// spreads their in-edges evenly over its lanes.
 __global__ __launch_bounds__(256) void k_gather_frontier(const int32_t* __restrict__ nbr,
                                                          const int32_t* __restrict__ L,
                                                          int32_t* __restrict__ adj, int64_t m,
-                                                         const uint32_t* __restrict__ chg,
+                                                         const int32_t* __restrict__ inv_pos,
+                                                         const PushEntry* __restrict__ ent,
                                                          const int32_t* __restrict__ fctl) {
-    if (fctl[1] == 0) {
+    if (!push_taken(fctl)) {
         gather_all(nbr, L, adj, m);
         return;
     }
-    const int64_t nvec = m >> 2;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const v4i32* nb4 = reinterpret_cast<const v4i32*>(nbr);
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += stride) {
-        const v4i32 u = __builtin_nontemporal_load(&nb4[i]);
-        const int64_t j = i << 2;
-        if ((chg[u.x >> 5] >> (u.x & 31)) & 1u) adj[j] = L[u.x];
-        if ((chg[u.y >> 5] >> (u.y & 31)) & 1u) adj[j + 1] = L[u.y];
-        if ((chg[u.z >> 5] >> (u.z & 31)) & 1u) adj[j + 2] = L[u.z];
-        if ((chg[u.w >> 5] >> (u.w & 31)) & 1u) adj[j + 3] = L[u.w];
-    }
-    const int64_t t = (nvec << 2) + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0 && t < m) {
-        const int32_t u = nbr[t];
-        if ((chg[u >> 5] >> (u & 31)) & 1u) adj[t] = L[u];
+    const int64_t nent = fctl[4];
+    const int lane = lane_id();
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    for (int64_t b0 = wave * 64; b0 < nent; b0 += nwaves * 64) {
+        PushEntry e{0, 0, 0, 0};
+        if (b0 + lane < nent) e = ent[b0 + lane];
+        const int32_t incl = wave_incl_sum(e.count);
+        const int32_t total = shfl_i32(incl, 63);
+        for (int32_t f0 = 0; f0 < total; f0 += 64) {
+            const int32_t f = f0 + lane;
+            // owner: the smallest lane whose inclusive sum exceeds f
+            int lo = 0, hi = 63;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const int mid = (lo + hi) >> 1;
+                if (shfl_i32(incl, mid) <= f) lo = mid + 1;
+                else hi = mid;
+            }
+            const int32_t o_incl = shfl_i32(incl, lo);
+            const int32_t o_cnt = shfl_i32(e.count, lo);
+            const int32_t o_beg = shfl_i32(e.begin, lo);
+            const int32_t o_lab = shfl_i32(e.label, lo);
+            if (f < total) adj[inv_pos[o_beg + f - (o_incl - o_cnt)]] = o_lab;
+        }
     }
 }
 
@@ -1291,9 +1513,10 @@ __global__ void k_iota(int32_t* p, int64_t count) {
 }
 // Sums and clears the step's change counters into the ring and advances
 // the step counter.  One workgroup of kChangeSlots threads.
-// With a frontier control block fctl = {all_active, mark, threshold}: the
-// next step recomputes everything unless this step marked, and marks when
-// this step changed at most `threshold` labels.
+// With a frontier control block fctl = {all_active, mark, threshold, ...}:
+// the next step recomputes everything unless this step marked, and marks
+// when this step changed at most `threshold` labels; the frontier gather's
+// words 3-5 are set up for the next step.
 __global__ __launch_bounds__(kChangeSlots) void k_end_step(unsigned long long* acc,
                                                            unsigned long long* ring, int32_t* ctr,
                                                            int32_t ring_size, int32_t* fctl) {
@@ -1315,6 +1538,10 @@ __global__ __launch_bounds__(kChangeSlots) void k_end_step(unsigned long long* a
             const int32_t marked = fctl[1];
             fctl[0] = marked ? 0 : 1;
             fctl[1] = sum <= (unsigned long long)fctl[2] ? 1 : 0;
+            fctl[3] = 1;  // a frontier gather has filled adj
+            fctl[4] = 0;
+            fctl[5] = 0;
+            fctl[6] = sum <= (unsigned long long)fctl[7] ? 1 : 0;
         }
     }
 }
@@ -1529,13 +1756,69 @@ int launch_pick(Method method, const StepArgs& a, int32_t* shown, hipStream_t s)
 int launch_gather(const StepArgs& a, int64_t m, int32_t* adj, hipStream_t s) {
     if (m == 0) return GLP_OK;
     const unsigned grid = (unsigned)std::min<int64_t>(grid_for(m / 4 + 1, 256), 256 * 64);
-    if (a.chg_r && a.fctl)
-        hipLaunchKernelGGL(k_gather_frontier, dim3(grid), dim3(256), 0, s, a.nbr, a.L, adj, m, a.chg_r,
-                           a.fctl);
-    else
-        hipLaunchKernelGGL(k_gather, dim3(grid), dim3(256), 0, s, a.nbr, a.L, adj, m);
+    hipLaunchKernelGGL(k_gather, dim3(grid), dim3(256), 0, s, a.nbr, a.L, adj, m);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "gather launch: %s", hipGetErrorString(e));
+}
+
+int launch_frontier_gather(const StepArgs& a, int64_t m, int32_t* adj, const InverseIndex& inv,
+                           void* push, hipStream_t s) {
+    if (m == 0) return GLP_OK;
+    if (!a.chg_r || !a.fctl || !inv.off || !inv.pos || !push)
+        return set_error(GLP_E_STATE, "frontier gather without its inverse index");
+    const int64_t nwords = ((int64_t)a.n + 31) / 32;
+    const int64_t nblocks = (nwords + 255) / 256;
+    auto* psum = static_cast<int2*>(push);
+    auto* ent = reinterpret_cast<PushEntry*>(psum + ((nblocks + 1) & ~(int64_t)1));
+    int32_t* fctl = const_cast<int32_t*>(a.fctl);
+    hipLaunchKernelGGL(k_push_count, dim3((unsigned)nblocks), dim3(256), 0, s, a.chg_r, nwords, inv.off,
+                       psum, a.fctl);
+    hipLaunchKernelGGL(k_push_scan, dim3(1), dim3(256), 0, s, psum, nblocks, fctl);
+    hipLaunchKernelGGL(k_push_write, dim3((unsigned)nblocks), dim3(256), 0, s, a.chg_r, nwords, inv.off,
+                       a.L, psum, ent, a.fctl);
+    const unsigned grid = (unsigned)std::min<int64_t>(grid_for(m / 4 + 1, 256), 256 * 64);
+    hipLaunchKernelGGL(k_gather_frontier, dim3(grid), dim3(256), 0, s, a.nbr, a.L, adj, m, inv.pos,
+                       ent, a.fctl);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "gather launch: %s", hipGetErrorString(e));
+}
+
+int32_t push_edge_limit(int64_t m) {
+    return (int32_t)std::min<int64_t>(m / kPushRatio, INT32_MAX);
+}
+
+size_t push_entry_bytes(int32_t n, int64_t m) {
+    // per-workgroup sums (int2, padded to 16 bytes), then the entries
+    const size_t nblocks = ((size_t)n / 32 + 1 + 255) / 256;
+    return 8 * ((nblocks + 1) & ~(size_t)1) + sizeof(PushEntry) * ((size_t)n + (size_t)(m / kPushChunk) + 64);
+}
+
+int inverse_index_build(const int32_t* nbr, int32_t n, int64_t m, int32_t* cursor, InverseIndex& inv,
+                        hipStream_t s) {
+    // counts into inv.off[0..n], exclusive scan, then the fill (cursor = copy
+    // of the offsets; n + 1 int32 of caller scratch)
+    const int64_t cnt = (int64_t)n + 1;
+    const int64_t nsums = (cnt + kScanBlock - 1) / kScanBlock;
+    int32_t* sums = nullptr;
+    GLP_HIP(hipMalloc(&sums, (size_t)nsums * 4));
+    int rc = launch_fill_i32(inv.off, 0, cnt, s);
+    const unsigned grid = (unsigned)std::min<int64_t>(grid_for(m, 256), 256 * 64);
+    if (!rc && m > 0) hipLaunchKernelGGL(k_inv_count, dim3(grid), dim3(256), 0, s, nbr, m, inv.off);
+    if (!rc) {
+        hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nsums), dim3(256), 0, s, inv.off, cnt, sums);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, sums, nsums);
+        hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nsums), dim3(256), 0, s, inv.off, cnt, sums);
+        hipError_t e = hipMemcpyAsync(cursor, inv.off, (size_t)cnt * 4, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) rc = set_error(GLP_E_HIP, "inverse index: %s", hipGetErrorString(e));
+    }
+    if (!rc && m > 0) hipLaunchKernelGGL(k_inv_fill, dim3(grid), dim3(256), 0, s, nbr, m, cursor, inv.pos);
+    if (!rc) {
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = set_error(GLP_E_HIP, "inverse index: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(sums);
+    return rc;
 }
 
 int launch_volume(const int32_t* L, int32_t* V, int32_t n, hipStream_t s) {
@@ -1567,6 +1850,35 @@ int launch_iota(int32_t* p, int64_t count, hipStream_t s) {
     hipLaunchKernelGGL(k_iota, dim3(grid), dim3(256), 0, s, p, count);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "iota launch: %s", hipGetErrorString(e));
+}
+
+// Random 4-byte reads of a table with uniform fmix32 indices (the label
+// gather's access pattern without its CSR stream): four independent reads
+// per lane per round, summed so the loads are kept.
+__global__ __launch_bounds__(256) void k_calib_read(const int32_t* __restrict__ T, uint32_t mask,
+                                                    int64_t reads, uint32_t seed,
+                                                    int32_t* __restrict__ out) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int32_t acc = 0;
+    for (int64_t j = tid * 4; j < reads; j += stride * 4) {
+        const uint32_t b = (uint32_t)j ^ seed;
+        acc += T[fmix32(b) & mask] + T[fmix32(b + 1u) & mask] + T[fmix32(b + 2u) & mask] +
+               T[fmix32(b + 3u) & mask];
+    }
+    out[tid] = acc;
+}
+
+int launch_calib_read(const int32_t* T, int64_t entries, int64_t reads, uint32_t seed, int32_t* out,
+                      int64_t out_entries, hipStream_t s) {
+    if (entries <= 0 || (entries & (entries - 1)) || entries > (int64_t)1 << 31)
+        return set_error(GLP_E_ARG, "table entries must be a power of two <= 2^31");
+    const int64_t grid = std::min<int64_t>(256 * 32, out_entries / 256);
+    if (grid < 1) return set_error(GLP_E_ARG, "calibration output too small");
+    hipLaunchKernelGGL(k_calib_read, dim3((unsigned)grid), dim3(256), 0, s, T, (uint32_t)(entries - 1),
+                       reads, seed, out);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "calibration launch: %s", hipGetErrorString(e));
 }
 
 int launch_end_step(unsigned long long* acc, unsigned long long* ring, int32_t* step_ctr,

@@ -23,9 +23,10 @@ struct glp_session {
     int32_t* mem = nullptr;              // SLPA memory (T+1) x n
     void* cm_scratch = nullptr;          // counting-method scratch
     uint8_t* act[2] = {nullptr, nullptr};  // frontier flags, double-buffered like L
-    int32_t* fctl = nullptr;             // frontier control {all_active, mark, threshold}
+    int32_t* fctl = nullptr;             // frontier control {all_active, mark, threshold, gather words}
     int64_t act_bytes = 0;
     uint32_t* chg[2] = {nullptr, nullptr};  // frontier gather: changed-vertex bitmaps
+    void* push = nullptr;                   // frontier gather: push entries of a step
     uint8_t* seeds = nullptr;              // seeded LP: n bytes, nonzero = seed
     bool seeds_set = false;
     int64_t chg_bytes = 0;

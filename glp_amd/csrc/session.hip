@@ -37,6 +37,8 @@ DeviceGraph::~DeviceGraph() {
     (void)hipFree(d_hub_base);
     (void)hipFree(d_hub_parts);
     (void)hipFree(d_hub_hist_base);
+    (void)hipFree(inv.off);
+    (void)hipFree(inv.pos);
 }
 
 template <class T>
@@ -206,6 +208,7 @@ glp_session::~glp_session() {
     (void)hipFree(act[1]);
     (void)hipFree(chg[0]);
     (void)hipFree(chg[1]);
+    (void)hipFree(push);
     (void)hipFree(seeds);
     (void)hipFree(fctl);
     (void)hipFree(hs.best);
@@ -231,7 +234,7 @@ size_t session_bytes(const HostCSR& g, const Plan& p, const glp_params& prm) {
     if (prm.method == GLP_USER) b += n * 4;
     if (prm.gather) b += (size_t)g.m * 4;
     if (prm.frontier) b += 2 * (n + 4);
-    if (prm.frontier && prm.gather) b += 2 * ((n + 31) / 32) * 4;
+    if (prm.frontier && prm.gather) b += 2 * ((n + 31) / 32) * 4 + push_entry_bytes(g.n, g.m);
     b += (size_t)p.pair_slots * 16 + (size_t)p.hist_slots * 4 + p.hub_chunks.size() * 4 +
          p.hubs.size() * 12;
     b += 8 * kChangeSlots + 8 * kChangeRing + 4;
@@ -284,11 +287,23 @@ int session_create(glp_graph* g, const glp_params* prm, glp_session** out) {
         s->act_bytes = ((int64_t)nn + 4) & ~(int64_t)3;
         GLP_HIP(hipMalloc(&s->act[0], (size_t)s->act_bytes));
         GLP_HIP(hipMalloc(&s->act[1], (size_t)s->act_bytes));
-        GLP_HIP(hipMalloc(&s->fctl, 4 * sizeof(int32_t)));
+        GLP_HIP(hipMalloc(&s->fctl, 8 * sizeof(int32_t)));
         if (prm->gather) {
             s->chg_bytes = (((int64_t)nn + 31) / 32) * 4;
             GLP_HIP(hipMalloc(&s->chg[0], (size_t)s->chg_bytes));
             GLP_HIP(hipMalloc(&s->chg[1], (size_t)s->chg_bytes));
+            GLP_HIP(hipMalloc(&s->push, push_entry_bytes(s->n, g->csr.m)));
+            DeviceGraph* dgm = g->dev;
+            if (!dgm->inv.off) {
+                // the push list doubles as the build's cursor scratch
+                const size_t m = std::max<size_t>((size_t)g->csr.m, 1);
+                GLP_HIP(hipMalloc(&dgm->inv.off, (nn + 1) * 4));
+                GLP_HIP(hipMalloc(&dgm->inv.pos, m * 4));
+                if ((rc = inverse_index_build(dgm->nbr, s->n, g->csr.m, static_cast<int32_t*>(s->push),
+                                              dgm->inv, s->stream)) != GLP_OK)
+                    return rc;
+                dgm->bytes += (nn + 1) * 4 + m * 4;
+            }
         }
     }
     GLP_HIP(hipMalloc(&s->d_acc, 8 * kChangeSlots));
@@ -335,7 +350,8 @@ int session_reset(glp_session* s) {
     if (s->fctl) {
         // the first step recomputes every vertex and does not mark; marking
         // starts after a step that changed at most n/32 labels
-        const int32_t ctl[4] = {1, 0, std::max<int32_t>(1, (int32_t)(n / 32)), 0};
+        const int32_t ctl[8] = {1, 0, std::max<int32_t>(1, (int32_t)(n / 32)), 0, 0, 0, 0,
+                                push_edge_limit(s->g->csr.m)};
         GLP_HIP(hipMemcpyAsync(s->fctl, ctl, sizeof ctl, hipMemcpyHostToDevice, s->stream));
         if ((rc = launch_zero_bytes(s->act[0], s->act_bytes, s->stream))) return rc;
         if ((rc = launch_zero_bytes(s->act[1], s->act_bytes, s->stream))) return rc;
@@ -430,7 +446,9 @@ int enqueue_step(glp_session* s, hipEvent_t* ev) {
     if (ev) (void)hipEventRecord(ev[2 * GLP_TIER_PICK + 1], s->stream);
     if (ev) (void)hipEventRecord(ev[2 * GLP_TIER_GATHER], s->stream);
     if (s->p.gather) {
-        if ((rc = launch_gather(a, s->g->csr.m, s->adj, s->stream))) return rc;
+        if (s->chg[0]) rc = launch_frontier_gather(a, s->g->csr.m, s->adj, s->g->dev->inv, s->push, s->stream);
+        else rc = launch_gather(a, s->g->csr.m, s->adj, s->stream);
+        if (rc) return rc;
     }
     if (ev) (void)hipEventRecord(ev[2 * GLP_TIER_GATHER + 1], s->stream);
     if ((rc = launch_step(s->method, s->p.weighted != 0, s->p.gather != 0, *s->g->dev, s->hs, a,
@@ -585,6 +603,49 @@ extern "C" int glp_device_count(int32_t* count) {
     hipError_t e = hipGetDeviceCount(&c);
     *count = e == hipSuccess ? c : 0;
     return GLP_OK;
+}
+
+extern "C" int glp_calib_random_read(int64_t table_entries, int64_t reads, int32_t repeats,
+                                     double* ms_best) {
+    clear_error();
+    if (!ms_best || reads <= 0 || repeats <= 0) return set_error(GLP_E_ARG, "bad calibration arguments");
+    if (table_entries <= 0 || (table_entries & (table_entries - 1)) || table_entries > (int64_t)1 << 31)
+        return set_error(GLP_E_ARG, "table entries must be a power of two <= 2^31");
+    int rc = ensure_device();
+    if (rc != GLP_OK) return rc;
+    const int64_t out_entries = 256 * 32 * 256;
+    int32_t* T = nullptr;
+    int32_t* out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    float best = 0.f;
+    auto fail = [&](hipError_t e) { return set_error(GLP_E_HIP, "calibration: %s", hipGetErrorString(e)); };
+    hipError_t e = hipMalloc(&T, (size_t)std::max<int64_t>(table_entries, 1) * 4);
+    if (e == hipSuccess) e = hipMalloc(&out, (size_t)out_entries * 4);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e != hipSuccess) rc = fail(e);
+    if (!rc) rc = launch_fill_i32(T, 1, table_entries, st);
+    // one untimed launch, then the best of `repeats`
+    for (int32_t r = 0; !rc && r <= repeats; ++r) {
+        if ((e = hipEventRecord(a, st)) != hipSuccess) { rc = fail(e); break; }
+        if ((rc = launch_calib_read(T, table_entries, reads, 11u + (uint32_t)r, out, out_entries, st))) break;
+        if ((e = hipEventRecord(b, st)) != hipSuccess || (e = hipEventSynchronize(b)) != hipSuccess) {
+            rc = fail(e);
+            break;
+        }
+        float ms = 0.f;
+        if ((e = hipEventElapsedTime(&ms, a, b)) != hipSuccess) { rc = fail(e); break; }
+        if (r > 0 && (best == 0.f || ms < best)) best = ms;
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(T);
+    (void)hipFree(out);
+    if (!rc) *ms_best = best;
+    return rc;
 }
 
 extern "C" int glp_session_create(glp_graph* g, const glp_params* p, glp_session** out) {

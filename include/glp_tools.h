@@ -38,6 +38,14 @@ int glp_tool_normalize(const char* bin_path, const char* out_path);
 int glp_tool_direct_edgelist2bin(const char* bin_path, const char* out_path,
                                  int32_t id_cutoff);
 
+/* No counterpart in the reference: measures the device's ceiling for the
+ * label gather's access pattern.  Issues `reads` random 4-byte reads of a
+ * table of table_entries int32 (a power of two <= 2^31; uniform fmix32
+ * indices, four independent reads per lane) in one launch, repeats the
+ * launch `repeats` times after one untimed launch, and stores the fastest
+ * launch's time in *ms_best.                                                 */
+int glp_calib_random_read(int64_t table_entries, int64_t reads, int32_t repeats, double* ms_best);
+
 /* Writes a graph back as a .bin: for every vertex u in id order, each
  * neighbour v > u in CSR order as an (u,v) pair (the inverse of
  * glp_graph_load_bin for undirected graphs without duplicates); weighted

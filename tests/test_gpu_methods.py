@@ -270,3 +270,11 @@ def test_server_budget_and_cancel():
     assert np.array_equal(got, alone)
     assert sv.poll(a).state == 3
     sv.close()
+
+
+def test_calib_random_read_runs():
+    """The random 4-byte read calibration (the label gather's access pattern)
+    runs and reports a time consistent with a sane read rate."""
+    ms = G.calib_random_read(1 << 20, 1 << 22, 2)
+    assert ms > 0
+    assert (1 << 22) / (ms / 1e3) > 1e8   # more than 0.1 G reads/s

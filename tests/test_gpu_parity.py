@@ -208,13 +208,26 @@ def test_frontier_equals_dense(scale, tiny):
     assert ch[-1] < (len(off) - 1) // 32   # the frontier was in use at the end
 
 
+def _push_steps(off, nbr, want):
+    """Steps (1-based) whose frontier gather takes the push form: from step 2
+    on, when the sources that changed in the step before have at most m/8
+    in-edges (the gather's own rule)."""
+    n, m = len(off) - 1, len(nbr)
+    indeg = np.bincount(nbr, minlength=n)
+    labs = [np.arange(n, dtype=np.int32)] + list(want)
+    return [t for t in range(2, len(labs))
+            if int(indeg[labs[t - 1] != labs[t - 2]].sum()) <= m // 8]
+
+
 @pytest.mark.parametrize("scale,tiny,weighted", [(14, False, False), (13, True, False),
-                                                 (13, False, True)])
+                                                 (13, False, True), (16, False, False)])
 def test_frontier_gather_equals_dense(scale, tiny, weighted):
-    """Separate gather with the frontier: after a step that changed at most
-    n/32 labels the gather rewrites only the edges whose source changed
-    (bitmap), otherwise all of them; every step must equal the oracle, also
-    under graph replay and across a checkpoint (which forces a full gather)."""
+    """Separate gather with the frontier: after a step whose changed sources
+    have few in-edges the gather pushes their labels through the inverse
+    index, otherwise it rewrites every edge; every step must equal the
+    oracle, also under graph replay and across a checkpoint (which forces a
+    full gather).  At scale 16 some source has more in-edges than one push
+    entry holds (2048)."""
     kw = dict(pack_max_deg=8, wave_max_deg=16, block_max_deg=32, hub_chunk=256,
               part_keys=256) if tiny else {}
     with G.tuning(**kw):
@@ -228,12 +241,14 @@ def test_frontier_gather_equals_dense(scale, tiny, weighted):
             s.step(8, graph=True)
             assert np.array_equal(s.labels(), want[10])
             state = s.save()
-            ch = s.changes()
         with G.Session(g, frontier=True, gather=True, weighted=weighted) as s2:
             s2.load(state)
             s2.step(5)
             assert np.array_equal(s2.labels(), want[15])
-    assert ch[-1] < (len(off) - 1) // 32   # the partial gather was in use
+    push = _push_steps(off, nbr, want)
+    assert len(push) >= 4 and push[-1] == 16, push   # the push form was in use
+    if scale == 16:
+        assert int(np.diff(off).max()) > 2048
 
 
 def test_frontier_rejected_for_layered():

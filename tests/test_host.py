@@ -278,3 +278,12 @@ def test_session_without_device_fails_cleanly():
     with pytest.raises(G.GLPError) as e:
         G.run(g, 1)
     assert e.value.code == -10
+
+
+@pytest.mark.parametrize("entries,reads", [(3, 1024), (0, 1024), (1 << 32, 1024), (1024, 0)])
+def test_calib_random_read_rejects_bad_arguments(entries, reads):
+    """The random-read calibration validates its arguments before touching
+    the device: the table must be a power of two <= 2^31 entries."""
+    with pytest.raises(G.GLPError) as e:
+        G.calib_random_read(entries, reads, 1)
+    assert e.value.code == -1
