@@ -45,12 +45,12 @@ __device__ __forceinline__ uint32_t kc_insert(unsigned long long* T, uint32_t ma
                                               uint32_t f) {
     uint32_t h = fmix32(key) & mask;
     const unsigned long long mine = (unsigned long long)key << 32;
+    // the claim is tried without a read first: one LDS operation when the
+    // slot is free, and no dearer than the read when it is not (measured
+    // faster on diverse and on converged steps alike)
     for (;;) {
-        unsigned long long cur = lds_load(&T[h]);
-        if (cur == 0ull) {
-            cur = atomicCAS(&T[h], 0ull, mine | f);
-            if (cur == 0ull) return h | kClaimed;
-        }
+        const unsigned long long cur = atomicCAS(&T[h], 0ull, mine | f);
+        if (cur == 0ull) return h | kClaimed;
         if ((uint32_t)(cur >> 32) == key) {
             atomicAdd(&T[h], (unsigned long long)f);
             return h;
@@ -677,6 +677,32 @@ __device__ __forceinline__ void block_body(const StepArgs& a, const int32_t* __r
 // at most 3/4 part_keys distinct labels (its table of 2 part_keys slots takes
 // part_keys before halving) even when every neighbour shows a different
 // label; once labels converge most buckets are empty and exit at once.
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ int32_t wave_incl_sum(int32_t x) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int32_t y = shfl_i32(x, lane >= d ? lane - d : lane);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+// Exclusive prefix of x over a workgroup of 256 threads; *total = the sum.
+__device__ __forceinline__ int32_t block_excl_sum(int32_t x, int32_t* total) {
+    __shared__ int32_t s_w[4];
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const int32_t incl = wave_incl_sum(x);
+    if (lane == 63) s_w[wv] = incl;
+    __syncthreads();
+    int32_t base = 0, all = 0;
+    for (int i = 0; i < 4; ++i) {
+        if (i < wv) base += s_w[i];
+        all += s_w[i];
+    }
+    __syncthreads();
+    *total = all;
+    return base + incl - x;
+}
 constexpr int kHubThreads = 256;
 
 // dynamic LDS of k_hub_agg: the chunk's table plus the bucket histogram
@@ -747,7 +773,6 @@ __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h
 // In-place exclusive scan of one hub's P x nch histogram (bucket-major);
 // also records the hub's pair total.
 __device__ __forceinline__ void hub_scan_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
-    __shared__ uint32_t s_part[kHubThreads];
     const int32_t v = h.hubs[blk];
     const int32_t d = a.off[v + 1] - a.off[v];
     const int64_t nch = (d + h.chunk - 1) / h.chunk;
@@ -757,25 +782,31 @@ __device__ __forceinline__ void hub_scan_body(const StepArgs& a, const HubArgs& 
     const int64_t per = (total + kHubThreads - 1) / kHubThreads;
     const int64_t lo = min((int64_t)threadIdx.x * per, total);
     const int64_t hi = min(lo + per, total);
+    // kR entries per round, loaded branch-free (clamped) so they are in
+    // flight together: the largest hubs' runs are hundreds of entries long
+    constexpr int kR = 64;
     uint32_t sum = 0u;
-    for (int64_t i = lo; i < hi; ++i) sum += hist[i];
-    s_part[threadIdx.x] = sum;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0u;
-        for (int i = 0; i < kHubThreads; ++i) {
-            const uint32_t x = s_part[i];
-            s_part[i] = run;
-            run += x;
-        }
-        h.hs.total[blk] = run;
+    for (int64_t i = lo; i < hi; i += kR) {
+        uint32_t x[kR];
+#pragma unroll
+        for (int k = 0; k < kR; ++k) x[k] = hist[min(i + k, hi - 1)];
+#pragma unroll
+        for (int k = 0; k < kR; ++k) sum += i + k < hi ? x[k] : 0u;
     }
-    __syncthreads();
-    uint32_t run = s_part[threadIdx.x];
-    for (int64_t i = lo; i < hi; ++i) {
-        const uint32_t x = hist[i];
-        hist[i] = run;
-        run += x;
+    int32_t pairs;
+    uint32_t run = (uint32_t)block_excl_sum((int32_t)sum, &pairs);
+    if (threadIdx.x == 0) h.hs.total[blk] = (uint32_t)pairs;
+    for (int64_t i = lo; i < hi; i += kR) {
+        uint32_t x[kR];
+#pragma unroll
+        for (int k = 0; k < kR; ++k) x[k] = hist[min(i + k, hi - 1)];
+#pragma unroll
+        for (int k = 0; k < kR; ++k) {
+            if (i + k < hi) {
+                hist[i + k] = run;
+                run += x[k];
+            }
+        }
     }
 }
 
@@ -809,28 +840,36 @@ __device__ __forceinline__ void part_pass(const unsigned long long* pr, uint32_t
                                           uint32_t limit, uint32_t* s_distinct, uint32_t* s_over) {
     const uint32_t mask = (1u << (32 - shift)) - 1u;
     constexpr int kU = 4;  // pairs loaded per thread before any is inserted
-    for (uint32_t i0 = threadIdx.x; i0 < n; i0 += kHubThreads * kU) {
-        if (lds_load(s_over)) return;
+    // the trip count is the wave's (first lane's), so the wave reduction
+    // below runs with every lane; lanes past n load a clamped pair and skip it
+    for (uint32_t w0 = threadIdx.x & ~63u; w0 < n; w0 += kHubThreads * kU) {
+        if (lds_load(s_over)) return;  // one LDS read for the wave: uniform
+        const uint32_t i0 = w0 + (uint32_t)lane_id();
         unsigned long long xs[kU];
         // clamped indices keep the loads branch-free (issued together)
 #pragma unroll
         for (int u = 0; u < kU; ++u) xs[u] = pr[min(i0 + u * kHubThreads, n - 1u)];
+        uint32_t claimed = 0u;
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            if (i0 + u * kHubThreads >= n) break;
             const uint32_t key = (uint32_t)(xs[u] >> 32);
             const uint32_t hk = fmix32(key);
-            if (hk < lo || hk >= hi) continue;
-            // a pair is already a (key << 32 | count) slot value
+            if (i0 + u * kHubThreads >= n || hk < lo || hk >= hi) continue;
+            // a pair is already a (key << 32 | count) slot value; most pairs
+            // of a bucket bring a new label while labels are diverse, so the
+            // claim is tried first
             uint32_t s = (hk * 0x9e3779b1u) >> shift;
-            for (;;) {
-                unsigned long long cur = lds_load(&T[s]);
+            for (uint32_t probes = 0;; ++probes) {
+                if (probes > mask) {
+                    // every slot is taken (waves that have not yet seen the
+                    // flag overshoot the limit): give the range up
+                    atomicExch(s_over, 1u);
+                    break;
+                }
+                const unsigned long long cur = atomicCAS(&T[s], 0ull, xs[u]);
                 if (cur == 0ull) {
-                    cur = atomicCAS(&T[s], 0ull, xs[u]);
-                    if (cur == 0ull) {
-                        if (atomicAdd(s_distinct, 1u) + 1u > limit) atomicExch(s_over, 1u);
-                        break;
-                    }
+                    ++claimed;
+                    break;
                 }
                 if ((uint32_t)(cur >> 32) == key) {
                     atomicAdd(&T[s], xs[u] & 0xffffffffull);
@@ -839,6 +878,12 @@ __device__ __forceinline__ void part_pass(const unsigned long long* pr, uint32_t
                 s = (s + 1u) & mask;
             }
         }
+        // one atomic per wave on the distinct-label counter
+        const int32_t wave_claimed = __builtin_amdgcn_readfirstlane(
+            shfl_i32(wave_incl_sum((int32_t)claimed), 63));
+        if (lane_id() == 0 && wave_claimed &&
+            atomicAdd(s_distinct, (uint32_t)wave_claimed) + (uint32_t)wave_claimed > limit)
+            atomicExch(s_over, 1u);
     }
 }
 
@@ -1229,31 +1274,6 @@ __global__ __launch_bounds__(256) void k_inv_fill(const int32_t* __restrict__ nb
 // of the sums in one workgroup, then each block rescans its run.
 constexpr int kScanPer = 16;                       // elements per thread
 constexpr int kScanBlock = 256 * kScanPer;         // elements per workgroup
-__device__ __forceinline__ int32_t wave_incl_sum(int32_t x) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int32_t y = shfl_i32(x, lane >= d ? lane - d : lane);
-        if (lane >= d) x += y;
-    }
-    return x;
-}
-// Exclusive prefix of x over the workgroup (256 threads); *total = the sum.
-__device__ __forceinline__ int32_t block_excl_sum(int32_t x, int32_t* total) {
-    __shared__ int32_t s_w[4];
-    const int lane = lane_id(), wv = threadIdx.x >> 6;
-    const int32_t incl = wave_incl_sum(x);
-    if (lane == 63) s_w[wv] = incl;
-    __syncthreads();
-    int32_t base = 0, all = 0;
-    for (int i = 0; i < 4; ++i) {
-        if (i < wv) base += s_w[i];
-        all += s_w[i];
-    }
-    __syncthreads();
-    *total = all;
-    return base + incl - x;
-}
 __global__ __launch_bounds__(256) void k_scan_sums(const int32_t* __restrict__ x, int64_t count,
                                                    int32_t* __restrict__ sums) {
     const int64_t b0 = (int64_t)blockIdx.x * kScanBlock + (int64_t)threadIdx.x * kScanPer;
