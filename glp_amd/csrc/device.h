@@ -43,8 +43,10 @@ struct Plan {
     Tuning tune;
     std::vector<int2> groups;                 // packed tier (edge-less groups last)
     int64_t groups_with_edges = 0;            // groups before the edge-less ones
-    std::vector<int32_t> half;                // wave tier, deg <= half_max_deg (half-wave each)
-    std::vector<int32_t> wave;                // wave tier, larger degrees
+    // wave-tier vertices as {vertex, first edge, end of edges, 0}: the edge
+    // range travels with the vertex, so a wave's loads skip the offsets
+    std::vector<int4> half;                   // deg <= half_max_deg (half-wave each)
+    std::vector<int4> wave;                   // larger degrees
     int32_t nclasses = 0;
     int32_t class_cap[kMaxBlockClasses] = {};  // LDS slots of each block class
     std::vector<int32_t> block[kMaxBlockClasses];
@@ -76,8 +78,8 @@ struct DeviceGraph {
     uint32_t* w = nullptr;
     Plan plan;
     int2* d_groups = nullptr;
-    int32_t* d_half = nullptr;
-    int32_t* d_wave = nullptr;
+    int4* d_half = nullptr;
+    int4* d_wave = nullptr;
     int32_t* d_block[kMaxBlockClasses] = {};
     int32_t* d_hubs = nullptr;
     int2* d_hub_chunks = nullptr;
@@ -146,8 +148,8 @@ enum { kLaunchPacked = 0, kLaunchHalf = 1, kLaunchWave = 2, kLaunchBlock0 = 3,
 struct JobDesc {
     StepArgs a;
     const int2* groups;
-    const int32_t* half;
-    const int32_t* wave;
+    const int4* half;
+    const int4* wave;
     const int32_t* block[kMaxBlockClasses];
     HubArgs hub;
     int32_t ngroups, nhalf, nwave;

@@ -281,44 +281,72 @@ __device__ __forceinline__ uint32_t packed_finish(const StepArgs& a, const Packe
     const int lane = lane_id();
     // key 0 is an unlabeled neighbour (seeded LP): it never counts
     const bool live = lane < s.nE && s.key != 0u;
-    wave_lds_sync();  // the previous group's reads of the table are done
-    t.key[lane] = 0ull;
-    t.key[lane + 64] = 0ull;
-    t.cnt[lane] = 0u;
-    t.cnt[lane + 64] = 0u;
-    t.best[lane] = 0ull;
-    wave_lds_sync();
-    uint32_t h = 0u;
-    bool claimed = false;  // this lane created the (owner, label) slot
-    if (live) {
-        const unsigned long long k64 = ((unsigned long long)(uint32_t)(s.own + 1) << 32) | s.key;
-        h = fmix32(s.key ^ ((uint32_t)s.own * 0x9e3779b9u)) & (kPackSlots - 1u);
-        for (;;) {
-            unsigned long long cur = lds_load(&t.key[h]);
-            if (cur == 0ull) {
-                cur = atomicCAS(&t.key[h], 0ull, k64);
-                if (cur == 0ull) {
-                    claimed = true;
-                    break;
-                }
-            }
-            if (cur == k64) break;
-            h = (h + 1u) & (kPackSlots - 1u);
+    uint64_t best = 0ull;  // on vertex lanes: the vertex's best packed score
+    bool decided = false;  // the whole group was decided without the table
+    if constexpr (count_score_of<H>::value && !W) {
+        // majority shortcut: a label on more than half of a vertex's counted
+        // edges wins outright.  Candidates per vertex: the labels of its
+        // first and of its last edge (one of them is the majority label
+        // unless both edges carry minority labels).
+        const unsigned long long livem = __ballot(live);
+        const int32_t sb = shfl_i32(s.vb, s.own), se = shfl_i32(s.ve, s.own);
+        const uint32_t o1 = shfl_u32(s.key, sb), o2 = shfl_u32(s.key, max(se - 1, 0));
+        const unsigned long long m1 = __ballot(live && s.key == o1);
+        const unsigned long long m2 = __ballot(live && s.key == o2);
+        const uint32_t w1 = shfl_u32(s.key, min(s.vb, 63)), w2 = shfl_u32(s.key, max(s.ve - 1, 0));
+        bool open = false;
+        if (lane < s.vc && !is_fixed<H>(a, s.v0 + lane)) {
+            const unsigned long long below = s.ve >= 64 ? ~0ull : ((1ull << s.ve) - 1ull);
+            const unsigned long long seg = s.vb >= 64 ? 0ull : below & ~((1ull << s.vb) - 1ull);
+            const uint32_t len = (uint32_t)__popcll(livem & seg);
+            const uint32_t c1 = (uint32_t)__popcll(m1 & seg), c2 = (uint32_t)__popcll(m2 & seg);
+            if (len == 0u) best = 0ull;  // no counted edge: the vertex keeps its label
+            else if (2u * c1 > len) best = H::label_score(a, s.v0 + lane, w1, c1);
+            else if (2u * c2 > len) best = H::label_score(a, s.v0 + lane, w2, c2);
+            else open = true;
         }
-        atomicAdd(&t.cnt[h], W ? s.fr : 1u);
+        decided = __ballot(open) == 0ull;  // wave-uniform
     }
-    wave_lds_sync();
-    // one score per distinct (owner, label): the slot's creator
-    if (claimed) {
-        const uint64_t p = H::label_score(a, s.v0 + s.own, s.key, t.cnt[h]);
-        atomicMax(&t.best[s.own], (unsigned long long)p);
+    if (!decided) {
+        wave_lds_sync();  // the previous group's reads of the table are done
+        t.key[lane] = 0ull;
+        t.key[lane + 64] = 0ull;
+        t.cnt[lane] = 0u;
+        t.cnt[lane + 64] = 0u;
+        t.best[lane] = 0ull;
+        wave_lds_sync();
+        uint32_t h = 0u;
+        bool claimed = false;  // this lane created the (owner, label) slot
+        if (live) {
+            const unsigned long long k64 = ((unsigned long long)(uint32_t)(s.own + 1) << 32) | s.key;
+            h = fmix32(s.key ^ ((uint32_t)s.own * 0x9e3779b9u)) & (kPackSlots - 1u);
+            for (;;) {
+                unsigned long long cur = lds_load(&t.key[h]);
+                if (cur == 0ull) {
+                    cur = atomicCAS(&t.key[h], 0ull, k64);
+                    if (cur == 0ull) {
+                        claimed = true;
+                        break;
+                    }
+                }
+                if (cur == k64) break;
+                h = (h + 1u) & (kPackSlots - 1u);
+            }
+            atomicAdd(&t.cnt[h], W ? s.fr : 1u);
+        }
+        wave_lds_sync();
+        // one score per distinct (owner, label): the slot's creator
+        if (claimed) {
+            const uint64_t p = H::label_score(a, s.v0 + s.own, s.key, t.cnt[h]);
+            atomicMax(&t.best[s.own], (unsigned long long)p);
+        }
+        wave_lds_sync();
+        best = t.best[lane];
     }
-    wave_lds_sync();
     bool changed = false;
     if (lane < s.vc && !is_fixed<H>(a, s.v0 + lane)) {
         const int32_t v = s.v0 + lane;
         // no edge, or no counted label: the vertex keeps its label
-        const uint64_t best = t.best[lane];
         const int32_t nl = best ? packed_label(best) : s.old;
         H::update_vertex(a, v, nl, best);
         changed = nl != s.old;
@@ -367,15 +395,13 @@ __device__ __forceinline__ void packed_body(const StepArgs& a, const int2* __res
 // loads issued before any is counted.
 constexpr uint32_t kHalfCap = 2 * kHalfMaxDeg;
 
-// Per-wave LDS of the half tier: one (key << 32 | sum) table per half and
-// each half's best packed score.
+// Per-wave LDS of the half tier: one (key << 32 | sum) table per half.
 struct HalfLds {
     unsigned long long slot[2][kHalfCap];
-    unsigned long long best[2];
 };
 
 template <class H, bool W, bool G, int NP>
-__device__ __forceinline__ void half_body(const StepArgs& a, const int32_t* __restrict__ verts,
+__device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restrict__ verts,
                                           int32_t nv, uint32_t blk) {
     __shared__ HalfLds s_half[4];
     const int lane = lane_id(), wv = threadIdx.x >> 6, hf = lane >> 5, hl = lane & 31;
@@ -387,14 +413,15 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int32_t* __re
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
         const int32_t idx = i0 + 2 * p + hf;
-        v[p] = verts[min(idx, nv - 1)];
+        const int4 ent = verts[min(idx, nv - 1)];
+        v[p] = ent.x;
+        b[p] = ent.y;
+        e[p] = ent.z;
         run[p] = idx < nv;
     }
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
         run[p] = run[p] && is_active<H>(a, v[p]) && !is_fixed<H>(a, v[p]);
-        b[p] = a.off[v[p]];
-        e[p] = a.off[v[p] + 1];
         old[p] = a.keep[v[p]];
     }
     // every tier vertex has >= 1 edge, so e-1 is a valid index; a vertex not
@@ -414,40 +441,48 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int32_t* __re
     for (int p = 0; p < NP; ++p) {
         if (__ballot(run[p]) == 0ull) continue;  // wave-uniform: both halves idle
         // majority shortcut (scores that are frequency sums): when the label
-        // of the half's first edge holds more than half of the half's counted
-        // frequency, no other label can reach its score
+        // of the half's first edge, or of its 17th, holds more than half of
+        // the half's counted frequency, no other label can reach its score
         uint64_t fast = 0ull;
+        bool peel = true;  // merge lanes of a shared label before the table
         if constexpr (count_score_of<H>::value) {
             const uint32_t k0 = key[p][0], k1 = key[p][1];
-            const uint32_t llo = (uint32_t)__builtin_amdgcn_readlane((int)k0, 0);
-            const uint32_t lhi = (uint32_t)__builtin_amdgcn_readlane((int)k0, 32);
-            const uint32_t lead = hf ? lhi : llo;
-            uint32_t c, tot;
+            const uint32_t lead = (uint32_t)__builtin_amdgcn_readlane((int)k0, hf ? 32 : 0);
+            const uint32_t lead2 = (uint32_t)__builtin_amdgcn_readlane((int)k0, hf ? 48 : 16);
+            uint32_t c, c2, tot;
             if (W) {
                 c = (k0 && k0 == lead ? fr[p][0] : 0u) + (k1 && k1 == lead ? fr[p][1] : 0u);
+                c2 = (k0 && k0 == lead2 ? fr[p][0] : 0u) + (k1 && k1 == lead2 ? fr[p][1] : 0u);
                 tot = (k0 ? fr[p][0] : 0u) + (k1 ? fr[p][1] : 0u);
 #pragma unroll
                 for (int d = 16; d >= 1; d >>= 1) {
                     c += shfl_u32(c, lane ^ d);
+                    c2 += shfl_u32(c2, lane ^ d);
                     tot += shfl_u32(tot, lane ^ d);
                 }
             } else {
-                const unsigned long long m0 = __ballot(k0 != 0u && k0 == lead);
-                const unsigned long long m1 = __ballot(k1 != 0u && k1 == lead);
-                const unsigned long long v0 = __ballot(k0 != 0u), v1 = __ballot(k1 != 0u);
                 const int sh = hf ? 32 : 0;
-                c = (uint32_t)(__popc((uint32_t)(m0 >> sh)) + __popc((uint32_t)(m1 >> sh)));
-                tot = (uint32_t)(__popc((uint32_t)(v0 >> sh)) + __popc((uint32_t)(v1 >> sh)));
+                auto half_count = [&](bool x0, bool x1) {
+                    return (uint32_t)(__popc((uint32_t)(__ballot(x0) >> sh)) +
+                                      __popc((uint32_t)(__ballot(x1) >> sh)));
+                };
+                c = half_count(k0 != 0u && k0 == lead, k1 != 0u && k1 == lead);
+                c2 = half_count(k0 != 0u && k0 == lead2, k1 != 0u && k1 == lead2);
+                tot = half_count(k0 != 0u, k1 != 0u);
             }
             if (run[p] && lead != 0u && 2ull * c > (unsigned long long)tot)
                 fast = H::label_score(a, v[p], lead, c);
+            else if (run[p] && lead2 != 0u && 2ull * c2 > (unsigned long long)tot)
+                fast = H::label_score(a, v[p], lead2, c2);
+            // neither candidate repeats much: the labels are diverse and
+            // merging lanes would find nothing to merge (wave-uniform)
+            peel = __ballot(run[p] && fast == 0ull && max(c, c2) >= 4u) != 0ull;
         }
         uint64_t best = fast;
         if (__ballot(run[p] && fast == 0ull) != 0ull) {  // wave-uniform: some half needs the table
             wave_lds_sync();  // the previous pair's reads are done
 #pragma unroll
             for (uint32_t i = 0; i < kHalfCap; i += 32) T[i + hl] = 0ull;
-            if (hl == 0) t.best[hf] = 0ull;
             wave_lds_sync();
             // per half, the lanes holding the half's first pending label
             // merge into one insert by the lowest of them (a converged vertex
@@ -459,6 +494,11 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int32_t* __re
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 uint32_t k = fast ? 0u : key[p][i], f = W ? fr[p][i] : 1u;
+                if (!peel) {
+                    sk[i] = k;
+                    hs[i] = k ? kc_insert(T, kHalfCap - 1u, k, f) : 0u;
+                    continue;
+                }
                 const unsigned long long rest = __ballot(k != 0u);
                 const uint32_t rlo = (uint32_t)rest, rhi = (uint32_t)(rest >> 32);
                 const int flo = rlo ? (int)__builtin_ctz(rlo) : 0;
@@ -489,9 +529,10 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int32_t* __re
             for (int i = 0; i < 2; ++i)
                 if (sk[i] && (hs[i] & kClaimed))
                     pm = max_u64(pm, H::label_score(a, v[p], sk[i], (uint32_t)T[hs[i] & ~kClaimed]));
-            if (pm) atomicMax(&t.best[hf], (unsigned long long)pm);
-            wave_lds_sync();
-            if (!fast) best = t.best[hf];
+            // max over the half (xor distances below 32 stay in the half)
+#pragma unroll
+            for (int d = 16; d >= 1; d >>= 1) pm = max_u64(pm, shfl_u64(pm, lane ^ d));
+            if (!fast) best = pm;
         }
         bool changed = false;
         if (hl == 0 && run[p]) {
@@ -514,7 +555,7 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int32_t* __re
 // wave's LDS table in turn.
 constexpr int kWaveNV = 4;
 template <class H, bool W, bool G>
-__device__ __forceinline__ void wave_body(const StepArgs& a, const int32_t* __restrict__ verts,
+__device__ __forceinline__ void wave_body(const StepArgs& a, const int4* __restrict__ verts,
                                           int32_t nv, uint32_t blk) {
     __shared__ unsigned long long s_tab[4][kWaveCap];  // (label key << 32 | count) slots
     const int lane = lane_id(), wv = threadIdx.x >> 6;
@@ -524,12 +565,15 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int32_t* __re
     bool run[kWaveNV];
     uint32_t key[kWaveNV][4], fr[kWaveNV][4];
 #pragma unroll
-    for (int k = 0; k < kWaveNV; ++k) v[k] = verts[min(i0 + k, nv - 1)];
+    for (int k = 0; k < kWaveNV; ++k) {
+        const int4 ent = verts[min(i0 + k, nv - 1)];
+        v[k] = ent.x;
+        b[k] = ent.y;
+        e[k] = ent.z;
+    }
 #pragma unroll
     for (int k = 0; k < kWaveNV; ++k) {
         run[k] = i0 + k < nv && is_active<H>(a, v[k]) && !is_fixed<H>(a, v[k]);
-        b[k] = a.off[v[k]];
-        e[k] = a.off[v[k] + 1];
         old[k] = a.keep[v[k]];
     }
     // every tier vertex has >= 1 edge, so e-1 is a valid index; a vertex not
@@ -549,30 +593,42 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int32_t* __re
         if (!run[k]) continue;  // wave-uniform
         uint64_t best = 0ull;
         bool decided = false;  // wave-uniform
+        bool peel = true;      // merge lanes of a shared label before the table
         if constexpr (count_score_of<H>::value) {
-            // majority shortcut: the label of lane 0's first edge wins outright
-            // when it holds more than half of the counted frequency
+            // majority shortcut: the label of the first edge, or of the
+            // 33rd, wins outright when it holds more than half of the
+            // counted frequency
             const uint32_t lead = (uint32_t)__builtin_amdgcn_readlane((int)key[k][0], 0);
-            uint32_t c = 0u, tot = 0u;
+            const uint32_t lead2 = (uint32_t)__builtin_amdgcn_readlane((int)key[k][0], 32);
+            uint32_t c = 0u, c2 = 0u, tot = 0u;
             if (W) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     c += key[k][i] && key[k][i] == lead ? fr[k][i] : 0u;
+                    c2 += key[k][i] && key[k][i] == lead2 ? fr[k][i] : 0u;
                     tot += key[k][i] ? fr[k][i] : 0u;
                 }
                 c = wave_sum_u32(c);
+                c2 = wave_sum_u32(c2);
                 tot = wave_sum_u32(tot);
             } else {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     c += (uint32_t)__popcll(__ballot(key[k][i] != 0u && key[k][i] == lead));
+                    c2 += (uint32_t)__popcll(__ballot(key[k][i] != 0u && key[k][i] == lead2));
                     tot += (uint32_t)__popcll(__ballot(key[k][i] != 0u));
                 }
             }
             if (lead != 0u && 2ull * c > (unsigned long long)tot) {
                 best = H::label_score(a, v[k], lead, c);
                 decided = true;
+            } else if (lead2 != 0u && 2ull * c2 > (unsigned long long)tot) {
+                best = H::label_score(a, v[k], lead2, c2);
+                decided = true;
             }
+            // neither candidate repeats much: the labels are diverse and
+            // merging lanes would find nothing to merge
+            peel = max(c, c2) >= 4u;
         }
         if (!decided) {
             const uint32_t cap = min(max(next_pow2(2u * (uint32_t)(e[k] - b[k])), 64u), (uint32_t)kWaveCap);
@@ -580,7 +636,10 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int32_t* __re
             for (uint32_t i = lane; i < cap; i += 64) T[i] = 0ull;
             wave_lds_sync();
 #pragma unroll
-            for (int i = 0; i < 4; ++i) wave_insert<W>(T, mask, key[k][i], fr[k][i]);
+            for (int i = 0; i < 4; ++i) {
+                if (peel) wave_insert<W>(T, mask, key[k][i], fr[k][i]);
+                else if (key[k][i]) kc_insert(T, mask, key[k][i], fr[k][i]);
+            }
             wave_lds_sync();
             for (uint32_t i = lane; i < cap; i += 64) {
                 const unsigned long long x = T[i];
@@ -1034,12 +1093,12 @@ __global__ __launch_bounds__(256) void k_packed(StepArgs a, const int2* __restri
     packed_body<H, W, G, NG>(a, groups, ngroups, nge, blockIdx.x);
 }
 template <class H, bool W, bool G, int NP>
-__global__ __launch_bounds__(256) void k_half(StepArgs a, const int32_t* __restrict__ verts,
+__global__ __launch_bounds__(256) void k_half(StepArgs a, const int4* __restrict__ verts,
                                               int32_t nv) {
     half_body<H, W, G, NP>(a, verts, nv, blockIdx.x);
 }
 template <class H, bool W, bool G>
-__global__ __launch_bounds__(256) void k_wave(StepArgs a, const int32_t* __restrict__ verts,
+__global__ __launch_bounds__(256) void k_wave(StepArgs a, const int4* __restrict__ verts,
                                               int32_t nv) {
     wave_body<H, W, G>(a, verts, nv, blockIdx.x);
 }

@@ -137,7 +137,8 @@ int build_plan(const HostCSR& g, const Tuning& t, Plan& p) {
         }
         close_group();
         if (d <= t.wave_max_deg) {
-            (d <= t.half_max_deg ? p.half : p.wave).push_back(v);
+            (d <= t.half_max_deg ? p.half : p.wave)
+                .push_back(make_int4(v, g.offsets[v], g.offsets[v + 1], 0));
             p.edges[1] += d;
             p.verts[1]++;
         } else if (d <= t.block_max_deg) {

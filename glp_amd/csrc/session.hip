@@ -53,7 +53,7 @@ static int upload(T** dst, const T* src, size_t count, hipStream_t s, size_t& by
 size_t device_graph_bytes(const HostCSR& g, const Plan& p) {
     size_t b = sizeof(int32_t) * ((size_t)g.n + 1) + sizeof(int32_t) * (size_t)g.m;
     if (g.weighted()) b += sizeof(uint32_t) * (size_t)g.m;
-    b += sizeof(int2) * p.groups.size() + sizeof(int32_t) * (p.half.size() + p.wave.size());
+    b += sizeof(int2) * p.groups.size() + sizeof(int4) * (p.half.size() + p.wave.size());
     for (int c = 0; c < p.nclasses; ++c) b += sizeof(int32_t) * p.block[c].size();
     b += sizeof(int32_t) * p.hubs.size() + sizeof(int2) * p.hub_chunks.size();
     b += (sizeof(int64_t) * 2 + sizeof(uint32_t)) * p.hubs.size();
