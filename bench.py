@@ -100,8 +100,8 @@ def bench_batched(args, G):
     log("mix: %d graphs, %d vertices, %d edges" % (len(graphs), n_tot, m_tot))
     b = G.Batch(graphs, method=G.CLASSIC)
     tuned_ms = b.autotune(3) if not args.no_autotune else None
-    chosen = {k: G.tuning_get(k) for k in ("pack_max_deg", "half_max_deg", "wave_max_deg", "block_max_deg",
-                                           "block_threads", "hub_chunk", "part_keys")}
+    chosen = {k: G.tuning_get(k) for k in ("pack_max_deg", "half_max_deg", "half_lanes", "wave_max_deg",
+                                           "block_max_deg", "block_threads", "hub_chunk", "part_keys")}
     log(("autotuned: %s" % chosen) if tuned_ms else "not autotuned")
     steps = args.steps
     # plain: one session per graph, stepped one graph after another

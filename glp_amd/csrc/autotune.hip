@@ -25,6 +25,7 @@ struct Candidates {
 const Candidates kSpace[] = {
     {"pack_max_deg", {16, 24, 32, 48, 64}},
     {"half_max_deg", {0, 48, 64}},
+    {"half_lanes", {0, 16, 32}},
     {"wave_max_deg", {64, 128, 192, 256}},
     {"block_max_deg", {512, 1024, 2048, 4096, 8192}},
     {"block_threads", {256, 512, 1024}},

@@ -21,7 +21,8 @@ namespace glp {
 // Degree thresholds and launch shapes; the autotuner writes these.
 struct Tuning {
     int32_t pack_max_deg = 32;     // deg <= this: lane-packed groups of <= 64 edges
-    int32_t half_max_deg = 64;     // deg <= this (<= 64): two vertices per wavefront (0: off)
+    int32_t half_max_deg = 64;     // deg <= this (<= 64): 64/half_lanes vertices per wavefront (0: off)
+    int32_t half_lanes = 0;        // lanes per vertex there: 16, 32, or 0 = chosen per launch
     int32_t wave_max_deg = 256;    // deg <= this: one wavefront per vertex (LDS table)
     int32_t block_max_deg = 4096;  // deg <= this: one workgroup per vertex (LDS table)
     int32_t hub_chunk = 2048;      // edges per aggregation workgroup for hubs (deg > block_max_deg)

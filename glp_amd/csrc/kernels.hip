@@ -388,31 +388,38 @@ __device__ __forceinline__ void packed_body(const StepArgs& a, const int2* __res
 }
 
 // ------------------------------------------------------------------ half ----
-// Wave-tier vertices of degree <= half_max_deg (<= kHalfMaxDeg): a wave
-// counts two at a time, lanes 0-31 one vertex and lanes 32-63 the other, two
-// edges per lane, each half in its own LDS table of kHalfCap slots.  A wave
-// takes NP such pairs (chosen per launch like the packed tier's NG), their
-// loads issued before any is counted.
+// Wave-tier vertices of degree <= half_max_deg (<= kHalfMaxDeg = 64): S lanes
+// count one vertex, so a wave counts 64/S at a time (S = 32: halves, two
+// edges per lane; S = 16: quarters, four edges per lane), each lane group in
+// its own LDS table of kHalfCap slots.  A wave takes NP such units (chosen
+// per launch like the packed tier's NG), their loads issued before any is
+// counted.
 constexpr uint32_t kHalfCap = 2 * kHalfMaxDeg;
 
-// Per-wave LDS of the half tier: one (key << 32 | sum) table per half.
+// Per-wave LDS of the half tier: one (key << 32 | sum) table per lane group.
+template <int S>
 struct HalfLds {
-    unsigned long long slot[2][kHalfCap];
+    unsigned long long slot[64 / S][kHalfCap];
 };
 
-template <class H, bool W, bool G, int NP>
+template <class H, bool W, bool G, int NP, int S = 32>
 __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restrict__ verts,
                                           int32_t nv, uint32_t blk) {
-    __shared__ HalfLds s_half[4];
-    const int lane = lane_id(), wv = threadIdx.x >> 6, hf = lane >> 5, hl = lane & 31;
-    const int32_t i0 = ((int32_t)blk * 4 + wv) * (2 * NP);
+    static_assert(S == 16 || S == 32, "16 or 32 lanes per vertex");
+    constexpr int V = 64 / S;               // vertices per unit
+    constexpr int EPL = kHalfMaxDeg / S;    // edges per lane
+    __shared__ HalfLds<S> s_half[4];
+    const int lane = lane_id(), wv = threadIdx.x >> 6, q = lane / S, ql = lane % S;
+    const int g0 = q * S;  // the group's first lane
+    const unsigned long long gm = ((1ull << S) - 1ull) << g0;  // the group's lanes
+    const int32_t i0 = ((int32_t)blk * 4 + wv) * (V * NP);
     if (i0 >= nv) return;  // wave-uniform
     int32_t v[NP], b[NP], e[NP], old[NP];
     bool run[NP];
-    uint32_t key[NP][2], fr[NP][2];
+    uint32_t key[NP][EPL], fr[NP][EPL];
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
-        const int32_t idx = i0 + 2 * p + hf;
+        const int32_t idx = i0 + V * p + q;
         const int4 ent = verts[min(idx, nv - 1)];
         v[p] = ent.x;
         b[p] = ent.y;
@@ -429,46 +436,62 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restr
 #pragma unroll
     for (int p = 0; p < NP; ++p)
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int32_t j = b[p] + hl + 32 * i;
+        for (int i = 0; i < EPL; ++i) {
+            const int32_t j = b[p] + ql + S * i;
             key[p][i] = H::template load_neighbor<W, G>(a, v[p], run[p] ? min(j, e[p] - 1) : b[p], &fr[p][i]);
             if (!run[p] || j >= e[p]) key[p][i] = 0u;
         }
-    HalfLds& t = s_half[wv];
-    unsigned long long* T = t.slot[hf];
+    unsigned long long* T = s_half[wv].slot[q];
+    // lanes of this group where x holds
+    auto group_count = [&](bool x) { return (uint32_t)__popcll(__ballot(x) & gm); };
+    // sum over this group's lanes (xor distances below S stay in the group)
+    auto group_sum = [&](uint32_t x) {
+#pragma unroll
+        for (int d = S / 2; d >= 1; d >>= 1) x += shfl_u32(x, lane ^ d);
+        return x;
+    };
     uint32_t nch = 0u;
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
-        if (__ballot(run[p]) == 0ull) continue;  // wave-uniform: both halves idle
+        if (__ballot(run[p]) == 0ull) continue;  // wave-uniform: every group idle
         // majority shortcut (scores that are frequency sums): when the label
-        // of the half's first edge, or of its 17th, holds more than half of
-        // the half's counted frequency, no other label can reach its score
+        // of the vertex's first edge, or of its (S/2+1)-th, holds more than
+        // half of the vertex's counted frequency, no other label can reach
+        // its score
         uint64_t fast = 0ull;
         bool peel = true;  // merge lanes of a shared label before the table
         if constexpr (count_score_of<H>::value) {
-            const uint32_t k0 = key[p][0], k1 = key[p][1];
-            const uint32_t lead = (uint32_t)__builtin_amdgcn_readlane((int)k0, hf ? 32 : 0);
-            const uint32_t lead2 = (uint32_t)__builtin_amdgcn_readlane((int)k0, hf ? 48 : 16);
-            uint32_t c, c2, tot;
+            const uint32_t lead = shfl_u32(key[p][0], g0);
+            const uint32_t lead2 = shfl_u32(key[p][0], g0 + S / 2);
+            uint32_t c = 0u, c2 = 0u, tot = 0u;
             if (W) {
-                c = (k0 && k0 == lead ? fr[p][0] : 0u) + (k1 && k1 == lead ? fr[p][1] : 0u);
-                c2 = (k0 && k0 == lead2 ? fr[p][0] : 0u) + (k1 && k1 == lead2 ? fr[p][1] : 0u);
-                tot = (k0 ? fr[p][0] : 0u) + (k1 ? fr[p][1] : 0u);
 #pragma unroll
-                for (int d = 16; d >= 1; d >>= 1) {
-                    c += shfl_u32(c, lane ^ d);
-                    c2 += shfl_u32(c2, lane ^ d);
-                    tot += shfl_u32(tot, lane ^ d);
+                for (int i = 0; i < EPL; ++i) {
+                    const uint32_t k = key[p][i];
+                    c += k && k == lead ? fr[p][i] : 0u;
+                    c2 += k && k == lead2 ? fr[p][i] : 0u;
+                    tot += k ? fr[p][i] : 0u;
                 }
+                c = group_sum(c);
+                c2 = group_sum(c2);
+                tot = group_sum(tot);
             } else {
-                const int sh = hf ? 32 : 0;
-                auto half_count = [&](bool x0, bool x1) {
-                    return (uint32_t)(__popc((uint32_t)(__ballot(x0) >> sh)) +
-                                      __popc((uint32_t)(__ballot(x1) >> sh)));
-                };
-                c = half_count(k0 != 0u && k0 == lead, k1 != 0u && k1 == lead);
-                c2 = half_count(k0 != 0u && k0 == lead2, k1 != 0u && k1 == lead2);
-                tot = half_count(k0 != 0u, k1 != 0u);
+                // only seeded LP has unlabeled neighbours (key 0); otherwise
+                // every edge of the vertex counts once
+                if constexpr (seeded_of<H>::value) {
+#pragma unroll
+                    for (int i = 0; i < EPL; ++i) tot += group_count(key[p][i] != 0u);
+                } else {
+                    tot = run[p] ? (uint32_t)(e[p] - b[p]) : 0u;
+                }
+#pragma unroll
+                for (int i = 0; i < EPL; ++i) c += group_count(key[p][i] != 0u && key[p][i] == lead);
+                // the second candidate only when some group still needs one
+                if (__ballot(run[p] && !(lead != 0u && 2ull * c > (unsigned long long)tot)) != 0ull) {
+#pragma unroll
+                    for (int i = 0; i < EPL; ++i)
+                        c2 += group_count(key[p][i] != 0u && key[p][i] == lead2);
+                }
             }
             if (run[p] && lead != 0u && 2ull * c > (unsigned long long)tot)
                 fast = H::label_score(a, v[p], lead, c);
@@ -479,46 +502,36 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restr
             peel = __ballot(run[p] && fast == 0ull && max(c, c2) >= 4u) != 0ull;
         }
         uint64_t best = fast;
-        if (__ballot(run[p] && fast == 0ull) != 0ull) {  // wave-uniform: some half needs the table
-            wave_lds_sync();  // the previous pair's reads are done
+        if (__ballot(run[p] && fast == 0ull) != 0ull) {  // wave-uniform: some group needs the table
+            wave_lds_sync();  // the previous unit's reads are done
 #pragma unroll
-            for (uint32_t i = 0; i < kHalfCap; i += 32) T[i + hl] = 0ull;
+            for (uint32_t i = 0; i < kHalfCap; i += S) T[i + ql] = 0ull;
             wave_lds_sync();
-            // per half, the lanes holding the half's first pending label
+            // per group, the lanes holding the group's first pending label
             // merge into one insert by the lowest of them (a converged vertex
             // shows one label on most edges, and same-address LDS atomics
             // serialise); the merged lanes neither insert nor score (the
-            // leader's slot holds the same count).  A half decided by the
+            // leader's slot holds the same count).  A vertex decided by the
             // shortcut inserts nothing.
-            uint32_t hs[2], sk[2];
+            uint32_t hs[EPL], sk[EPL];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
+            for (int i = 0; i < EPL; ++i) {
                 uint32_t k = fast ? 0u : key[p][i], f = W ? fr[p][i] : 1u;
-                if (!peel) {
-                    sk[i] = k;
-                    hs[i] = k ? kc_insert(T, kHalfCap - 1u, k, f) : 0u;
-                    continue;
+                if (peel) {
+                    const unsigned long long rest = __ballot(k != 0u) & gm;
+                    const int first = rest ? (int)__builtin_ctzll(rest) : g0;
+                    const uint32_t lead = shfl_u32(k, first);
+                    const bool same = k != 0u && k == lead;
+                    const uint32_t cnt = group_count(same);
+                    if (W) {
+                        // weighted: the group's sum of the merged lanes' frequencies
+                        const uint32_t x = group_sum(same ? f : 0u);
+                        if (same && cnt >= 2u) f = x;
+                    } else if (same && cnt >= 2u) {
+                        f = cnt;
+                    }
+                    if (same && cnt >= 2u && lane != first) k = 0u;  // merged into the leader
                 }
-                const unsigned long long rest = __ballot(k != 0u);
-                const uint32_t rlo = (uint32_t)rest, rhi = (uint32_t)(rest >> 32);
-                const int flo = rlo ? (int)__builtin_ctz(rlo) : 0;
-                const int fhi = rhi ? 32 + (int)__builtin_ctz(rhi) : 32;
-                const uint32_t llo = rlo ? (uint32_t)__builtin_amdgcn_readlane((int)k, flo) : 0u;
-                const uint32_t lhi = rhi ? (uint32_t)__builtin_amdgcn_readlane((int)k, fhi) : 0u;
-                const uint32_t lead = hf ? lhi : llo;
-                const bool same = k != 0u && k == lead;
-                const unsigned long long sm = __ballot(same);
-                const uint32_t cnt = (uint32_t)__popc(hf ? (uint32_t)(sm >> 32) : (uint32_t)sm);
-                if (W) {
-                    // weighted: the half's sum of the merged lanes' frequencies
-                    uint32_t x = same ? f : 0u;
-#pragma unroll
-                    for (int d = 16; d >= 1; d >>= 1) x += shfl_u32(x, lane ^ d);
-                    if (same && cnt >= 2u) f = x;
-                } else if (same && cnt >= 2u) {
-                    f = cnt;
-                }
-                if (same && cnt >= 2u && lane != (hf ? fhi : flo)) k = 0u;  // merged into the leader
                 sk[i] = k;
                 hs[i] = k ? kc_insert(T, kHalfCap - 1u, k, f) : 0u;
             }
@@ -526,24 +539,24 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restr
             // one score per distinct label: the slot's creator
             uint64_t pm = 0ull;
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < EPL; ++i)
                 if (sk[i] && (hs[i] & kClaimed))
                     pm = max_u64(pm, H::label_score(a, v[p], sk[i], (uint32_t)T[hs[i] & ~kClaimed]));
-            // max over the half (xor distances below 32 stay in the half)
+            // max over the group (xor distances below S stay in the group)
 #pragma unroll
-            for (int d = 16; d >= 1; d >>= 1) pm = max_u64(pm, shfl_u64(pm, lane ^ d));
+            for (int d = S / 2; d >= 1; d >>= 1) pm = max_u64(pm, shfl_u64(pm, lane ^ d));
             if (!fast) best = pm;
         }
         bool changed = false;
-        if (hl == 0 && run[p]) {
+        if (ql == 0 && run[p]) {
             const int32_t nl = best ? packed_label(best) : old[p];
             H::update_vertex(a, v[p], nl, best);
             changed = nl != old[p];
             if (changed) note_changed(a, v[p]);
         }
         nch += (uint32_t)__popcll(__ballot(changed));
-        if (marking<H>(a) && shfl_i32(changed ? 1 : 0, lane & 32) != 0)
-            mark_vertex(a, v[p], b[p], e[p], hl, 32);
+        if (marking<H>(a) && shfl_i32(changed ? 1 : 0, g0) != 0)
+            mark_vertex(a, v[p], b[p], e[p], ql, S);
     }
     add_changes(a, nch);
 }
@@ -612,11 +625,21 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int4* __restr
                 c2 = wave_sum_u32(c2);
                 tot = wave_sum_u32(tot);
             } else {
+                // only seeded LP has unlabeled neighbours (key 0); otherwise
+                // every edge of the vertex counts once
+                if constexpr (seeded_of<H>::value) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
+                    for (int i = 0; i < 4; ++i) tot += (uint32_t)__popcll(__ballot(key[k][i] != 0u));
+                } else {
+                    tot = (uint32_t)(e[k] - b[k]);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
                     c += (uint32_t)__popcll(__ballot(key[k][i] != 0u && key[k][i] == lead));
-                    c2 += (uint32_t)__popcll(__ballot(key[k][i] != 0u && key[k][i] == lead2));
-                    tot += (uint32_t)__popcll(__ballot(key[k][i] != 0u));
+                if (!(lead != 0u && 2ull * c > (unsigned long long)tot)) {  // wave-uniform
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        c2 += (uint32_t)__popcll(__ballot(key[k][i] != 0u && key[k][i] == lead2));
                 }
             }
             if (lead != 0u && 2ull * c > (unsigned long long)tot) {
@@ -1092,10 +1115,10 @@ __global__ __launch_bounds__(256) void k_packed(StepArgs a, const int2* __restri
                                                 int32_t ngroups, int32_t nge) {
     packed_body<H, W, G, NG>(a, groups, ngroups, nge, blockIdx.x);
 }
-template <class H, bool W, bool G, int NP>
+template <class H, bool W, bool G, int NP, int S>
 __global__ __launch_bounds__(256) void k_half(StepArgs a, const int4* __restrict__ verts,
                                               int32_t nv) {
-    half_body<H, W, G, NP>(a, verts, nv, blockIdx.x);
+    half_body<H, W, G, NP, S>(a, verts, nv, blockIdx.x);
 }
 template <class H, bool W, bool G>
 __global__ __launch_bounds__(256) void k_wave(StepArgs a, const int4* __restrict__ verts,
@@ -1640,6 +1663,26 @@ static inline unsigned grid_for(int64_t work, int per_block) {
     return (unsigned)std::max<int64_t>(1, g);
 }
 
+// The half tier with S lanes per vertex: a unit is 64/S vertices, and a
+// wave takes 4, 2 or 1 units (units_per_wave).
+template <class H, bool W, bool G, int S>
+static void launch_half(const StepArgs& a, const int4* verts, int32_t nh, hipStream_t s) {
+    constexpr int V = 64 / S;
+    switch (units_per_wave((nh + V - 1) / V)) {
+        case 4:
+            hipLaunchKernelGGL((k_half<H, W, G, 4, S>), dim3(grid_for(nh, 16 * V)), dim3(256), 0, s, a,
+                               verts, nh);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_half<H, W, G, 2, S>), dim3(grid_for(nh, 8 * V)), dim3(256), 0, s, a,
+                               verts, nh);
+            break;
+        default:
+            hipLaunchKernelGGL((k_half<H, W, G, 1, S>), dim3(grid_for(nh, 4 * V)), dim3(256), 0, s, a,
+                               verts, nh);
+    }
+}
+
 template <class H, bool W, bool G>
 static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepArgs& a,
                         hipStream_t s, hipEvent_t* ev) {
@@ -1669,20 +1712,13 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
     mark(GLP_TIER_WAVE, 0);
     if (!p.half.empty()) {
         const int32_t nh = (int32_t)p.half.size();
-        // a pair of vertices per wave is one unit
-        switch (units_per_wave((nh + 1) / 2)) {
-            case 4:
-                hipLaunchKernelGGL((k_half<H, W, G, 4>), dim3(grid_for(nh, 32)), dim3(256), 0, s, a,
-                                   g.d_half, nh);
-                break;
-            case 2:
-                hipLaunchKernelGGL((k_half<H, W, G, 2>), dim3(grid_for(nh, 16)), dim3(256), 0, s, a,
-                                   g.d_half, nh);
-                break;
-            default:
-                hipLaunchKernelGGL((k_half<H, W, G, 1>), dim3(grid_for(nh, 8)), dim3(256), 0, s, a,
-                                   g.d_half, nh);
-        }
+        // 16 lanes per vertex halve the instructions per vertex and double
+        // the vertices in flight per wave; they pay off once the launch has
+        // enough vertices to fill the chip with waves of four units
+        const bool quarter = p.tune.half_lanes == 16 ||
+                             (p.tune.half_lanes == 0 && nh / 16 >= kFillWaves);
+        if (quarter) launch_half<H, W, G, 16>(a, g.d_half, nh, s);
+        else launch_half<H, W, G, 32>(a, g.d_half, nh, s);
     }
     if (!p.wave.empty()) {
         const int32_t nw = (int32_t)p.wave.size();

@@ -26,6 +26,7 @@ struct TuningField {
 };
 const TuningField kFields[] = {
     {"pack_max_deg", &Tuning::pack_max_deg}, {"half_max_deg", &Tuning::half_max_deg},
+    {"half_lanes", &Tuning::half_lanes},
     {"wave_max_deg", &Tuning::wave_max_deg},
     {"block_max_deg", &Tuning::block_max_deg}, {"hub_chunk", &Tuning::hub_chunk},
     {"part_keys", &Tuning::part_keys},       {"part_limit", &Tuning::part_limit},
@@ -76,6 +77,8 @@ static int check_tuning(const Tuning& t) {
     if (t.pack_max_deg < 0 || t.pack_max_deg > 64) return set_error(GLP_E_ARG, "pack_max_deg must be in [0,64]");
     if (t.half_max_deg < 0 || t.half_max_deg > kHalfMaxDeg)
         return set_error(GLP_E_ARG, "half_max_deg must be in [0, %d]", kHalfMaxDeg);
+    if (t.half_lanes != 0 && t.half_lanes != 16 && t.half_lanes != 32)
+        return set_error(GLP_E_ARG, "half_lanes must be 0 (chosen per launch), 16 or 32");
     if (t.wave_max_deg < t.pack_max_deg || t.wave_max_deg > kWaveCap / 2)
         return set_error(GLP_E_ARG, "wave_max_deg must be in [pack_max_deg, %d]", kWaveCap / 2);
     // the largest LDS table (2*deg slots of 8 bytes) must fit a CU's 160 KB

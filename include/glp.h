@@ -102,9 +102,11 @@ int glp_device_count(int32_t* count);
  * GLP_TUNING environment variable names a key=value file read at first use).
  * Replaces the compile-time constants of prop/classicLprop.cuh:178-198,
  * 235, 302, 311-316, 381 (tier thresholds, block sizes, LDS table sizes).
- * Keys: pack_max_deg, half_max_deg (0 turns the two-vertices-per-wavefront
- * part of the wave tier off), wave_max_deg, block_max_deg, hub_chunk,
- * part_keys, part_limit, block_threads.                                      */
+ * Keys: pack_max_deg, half_max_deg (0 turns the several-vertices-per-
+ * wavefront part of the wave tier off), half_lanes (16 or 32 lanes per
+ * vertex there; 0 chooses per launch from the amount of work),
+ * wave_max_deg, block_max_deg, hub_chunk, part_keys, part_limit,
+ * block_threads.                                                             */
 int glp_tuning_set(const char* key, int32_t value);
 int glp_tuning_get(const char* key, int32_t* value);
 /* Autotuner (no reference counterpart; replaces hand-picked constants):

@@ -84,13 +84,20 @@ def test_classic_tier_boundaries():
     assert_steps_equal(gpu_steps(g, 5), O.classic(off, nbr, 5))
 
 
-@pytest.mark.parametrize("half,pack", [(0, 32), (48, 32), (64, 32), (64, 0)])
-def test_half_wave_tier(half, pack):
-    """Degrees in (pack_max_deg, half_max_deg] run two per wavefront (one per
-    32-lane half); half_max_deg=0 sends them through the one-vertex wave path.
-    pack_max_deg=0 puts degrees 1..64 in the half tier.  Classic unweighted and
-    weighted, and layered, must match the oracle either way."""
-    with G.tuning(pack_max_deg=pack, half_max_deg=half):
+@pytest.mark.parametrize("half,pack,lanes", [(0, 32, 16), (48, 32, 16), (64, 32, 16), (64, 0, 16),
+                                             (48, 32, 32), (64, 32, 32), (64, 0, 32)])
+def test_half_wave_tier(half, pack, lanes):
+    """Degrees in (pack_max_deg, half_max_deg] run 64/lanes per wavefront
+    (16 or 32 lanes per vertex); half_max_deg=0 sends them through the
+    one-vertex wave path.  pack_max_deg=0 puts degrees 1..64 in the half
+    tier.  Classic unweighted and weighted, and layered, must match the
+    oracle either way; a planted-partition graph run to convergence takes
+    the majority shortcut and the lane merging."""
+    with G.tuning(pack_max_deg=pack, half_max_deg=half, half_lanes=lanes):
+        gp = G.Graph.planted(6000, 6, 28.0, 8.0, seed=35)
+        offp, nbrp, _ = gp.csr()
+        _, dumpp, _ = G.run(gp, 12, dump=True)
+        assert_steps_equal(dumpp, O.classic(offp, nbrp, 12))
         g = G.Graph.rmat(14, 16, seed=33)
         off, nbr, _ = g.csr()
         _, dump, timing = G.run(g, 6, dump=True)
