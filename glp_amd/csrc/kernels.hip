@@ -907,10 +907,19 @@ __device__ __forceinline__ void hub_scatter_body(const StepArgs& a, const HubArg
     __syncthreads();
     const unsigned long long* src = h.hs.pairs + h.base[hc.x] + (int64_t)hc.y * h.chunk;
     unsigned long long* dst = h.hs.sorted + h.base[hc.x];
-    for (uint32_t i = threadIdx.x; i < n; i += kHubThreads) {
-        const unsigned long long pr = src[i];
-        const uint32_t q = bucket_of(fmix32((uint32_t)(pr >> 32)), P);
-        dst[atomicAdd(&s_cur[q], 1u)] = pr;
+    // kR pairs per thread loaded (clamped, branch-free) before any is placed,
+    // so their loads are in flight together
+    constexpr int kR = 8;
+    for (uint32_t i0 = threadIdx.x; i0 < n; i0 += kHubThreads * kR) {
+        unsigned long long pr[kR];
+#pragma unroll
+        for (int r = 0; r < kR; ++r) pr[r] = src[min(i0 + r * kHubThreads, n - 1u)];
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+            if (i0 + r * kHubThreads >= n) break;
+            const uint32_t q = bucket_of(fmix32((uint32_t)(pr[r] >> 32)), P);
+            dst[atomicAdd(&s_cur[q], 1u)] = pr[r];
+        }
     }
 }
 
