@@ -90,6 +90,7 @@ _sig("glp_device_count", ctypes.c_int, _c_i32p)
 _sig("glp_graph_load_bin", ctypes.c_int, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, _pp)
 _sig("glp_graph_from_csr", ctypes.c_int, ctypes.c_int32, ctypes.c_int64, _c_i32p, _c_i32p, _c_u32p, _pp)
 _sig("glp_graph_info", ctypes.c_int, _vp, _c_i32p, _c_i64p, _c_i32p, _c_i32p)
+_sig("glp_graph_symmetric", ctypes.c_int, _vp, _c_i32p)
 _sig("glp_graph_csr", ctypes.c_int, _vp, _c_i32p, _c_i32p, _c_u32p)
 _sig("glp_graph_free", None, _vp)
 _sig("glp_graph_save_bin", ctypes.c_int, _vp, ctypes.c_char_p)
@@ -329,6 +330,12 @@ class Graph:
         _check(_lib.glp_graph_info(self._h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(d),
                                    ctypes.byref(w)))
         return {"n": n.value, "m": m.value, "max_degree": d.value, "weighted": bool(w.value)}
+
+    def symmetric(self):
+        """Whether every edge u->v has an edge v->u (the frontier needs it)."""
+        x = ctypes.c_int32()
+        _check(_lib.glp_graph_symmetric(self._h, ctypes.byref(x)))
+        return bool(x.value)
 
     @property
     def n(self):

@@ -87,6 +87,9 @@ extern "C" int glp_batch_create(int32_t ngraphs, glp_graph* const* graphs, const
         eo += g.m;
     }
     u.finish();
+    // a disjoint union is symmetric when every part is
+    u.symmetric = 1;
+    for (int32_t i = 0; i < ngraphs; ++i) u.symmetric = std::min(u.symmetric, graphs[i]->csr.symmetric);
     int rc = session_create(b->uni.get(), p, &b->sess);
     if (rc != GLP_OK) return rc;
     *out = b.release();

@@ -135,6 +135,7 @@ extern "C" int glp_gen_rmat(int32_t scale, int32_t edge_factor, double a, double
     auto g = std::make_unique<glp_graph>();
     int rc = csr_from_pairs(n, pairs, threads, g->csr);
     if (rc != GLP_OK) return rc;
+    g->csr.symmetric = 1;  // every pair was added in both directions
     if (wmax > 0) {
         // symmetric weights: w(u,v) = w(v,u), a hash of the unordered pair
         HostCSR& cs = g->csr;
@@ -245,6 +246,7 @@ extern "C" int glp_gen_planted(int32_t n, int32_t k, double deg_in, double deg_o
     auto g = std::make_unique<glp_graph>();
     int rc = csr_from_pairs(n, pairs, threads, g->csr);
     if (rc != GLP_OK) return rc;
+    g->csr.symmetric = 1;  // every pair was added in both directions
     if (community_out)
         for (int32_t c = 0; c < k; ++c)
             for (int64_t v = start[c]; v < start[c + 1]; ++v) community_out[v] = c;
@@ -347,6 +349,7 @@ extern "C" int glp_gen_lfr(int32_t n, double tau1, int32_t kmin, int32_t kmax, d
     auto g = std::make_unique<glp_graph>();
     int rc = csr_from_pairs(n, pairs, 0, g->csr);
     if (rc != GLP_OK) return rc;
+    g->csr.symmetric = 1;  // every pair was added in both directions
     if (community_out) std::memcpy(community_out, comm.data(), sizeof(int32_t) * n);
     *out = g.release();
     return GLP_OK;

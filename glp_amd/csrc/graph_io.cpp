@@ -114,7 +114,41 @@ static int build_from_records(const int32_t* rec, size_t nrec, int rec_words,
         }
     }
     out.finish();
+    out.symmetric = directed ? -1 : 1;
     return GLP_OK;
+}
+
+bool csr_is_symmetric(HostCSR& g) {
+    if (g.symmetric >= 0) return g.symmetric == 1;
+    const int32_t n = g.n;
+    std::vector<int64_t> toff((size_t)n + 1, 0);
+    for (int64_t j = 0; j < g.m; ++j) toff[(size_t)g.neighbors[j] + 1]++;
+    for (int32_t v = 0; v < n; ++v) toff[v + 1] += toff[v];
+    std::vector<int32_t> tin((size_t)g.m);
+    {
+        std::vector<int64_t> cur(toff.begin(), toff.end() - 1);
+        for (int32_t u = 0; u < n; ++u)
+            for (int32_t j = g.offsets[u]; j < g.offsets[u + 1]; ++j) tin[cur[g.neighbors[j]]++] = u;
+    }
+    int ok = 1;
+    const int threads = host_threads(0);
+#pragma omp parallel num_threads(threads) reduction(&& : ok)
+    {
+        std::vector<int32_t> out;
+#pragma omp for schedule(dynamic, 4096)
+        for (int32_t v = 0; v < n; ++v) {
+            if (!ok) continue;
+            out.assign(g.neighbors.begin() + g.offsets[v], g.neighbors.begin() + g.offsets[v + 1]);
+            std::sort(out.begin(), out.end());
+            for (int64_t k = toff[v]; k < toff[v + 1]; ++k)
+                if (!std::binary_search(out.begin(), out.end(), tin[k])) {
+                    ok = 0;
+                    break;
+                }
+        }
+    }
+    g.symmetric = ok ? 1 : 0;
+    return ok != 0;
 }
 
 int csr_from_pairs(int32_t n, std::vector<uint64_t>& pairs, int threads, HostCSR& out) {
@@ -227,6 +261,13 @@ extern "C" int glp_graph_from_csr(int32_t n, int64_t m, const int32_t* offsets,
     return GLP_OK;
 }
 
+extern "C" int glp_graph_symmetric(glp_graph* g, int32_t* symmetric) {
+    clear_error();
+    if (!g || !symmetric) return set_error(GLP_E_ARG, "null argument");
+    *symmetric = csr_is_symmetric(g->csr) ? 1 : 0;
+    return GLP_OK;
+}
+
 extern "C" int glp_graph_info(const glp_graph* g, int32_t* n, int64_t* m,
                               int32_t* max_degree, int32_t* weighted) {
     if (!g) return set_error(GLP_E_ARG, "null graph");
@@ -314,6 +355,7 @@ extern "C" int glp_graph_degree_sort(const glp_graph* g, glp_graph** out, int32_
         }
     }
     d.finish();
+    d.symmetric = c.symmetric;  // a relabelling keeps (a)symmetry
     if (perm_out) std::memcpy(perm_out, perm.data(), sizeof(int32_t) * n);
     *out = ng.release();
     return GLP_OK;

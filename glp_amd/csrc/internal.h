@@ -26,6 +26,9 @@ struct HostCSR {
     std::vector<int32_t> neighbors;
     std::vector<uint32_t> weights;
     int32_t max_degree = 0;
+    // 1: every edge u->v has an edge v->u (the frontier marks a changed
+    // vertex's out-neighbours, so it needs this); 0: not; -1: not yet known
+    int8_t symmetric = -1;
     bool weighted() const { return !weights.empty(); }
     void finish();  // computes max_degree
 };
@@ -70,5 +73,9 @@ int host_threads(int requested);
 // pairs (u,v) packed as (u<<32|v); self-loops must already be absent.
 // pairs is consumed (sorted in place).
 int csr_from_pairs(int32_t n, std::vector<uint64_t>& pairs, int threads, HostCSR& out);
+// Whether every edge u->v has an edge v->u (multiplicities may differ);
+// computed once (transpose by counting, then each in-list looked up in the
+// sorted out-list) and kept in g.symmetric.
+bool csr_is_symmetric(HostCSR& g);
 
 }  // namespace glp

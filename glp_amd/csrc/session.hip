@@ -245,6 +245,10 @@ int session_create(glp_graph* g, const glp_params* prm, glp_session** out) {
     int rc = ensure_device();
     if (rc != GLP_OK) return rc;
     if ((rc = check_params(g->csr, prm)) != GLP_OK) return rc;
+    if (prm->frontier && !csr_is_symmetric(g->csr))
+        return set_error(GLP_E_UNSUPPORTED,
+                         "the frontier needs a symmetric graph (every edge u->v with an edge v->u): "
+                         "it marks a changed vertex's out-neighbours for recomputation");
     init_kernels();
     auto s = std::make_unique<glp_session>();
     s->g = g;

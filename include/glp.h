@@ -134,6 +134,12 @@ int glp_graph_from_csr(int32_t n, int64_t m, const int32_t* offsets,
 /* Shape of a graph: n (CSRGraph::n), m (CSRGraph::m), max degree, weighted. */
 int glp_graph_info(const glp_graph* g, int32_t* n, int64_t* m,
                    int32_t* max_degree, int32_t* weighted);
+/* No counterpart in the reference (its frontier code assumes undirected
+ * input): *symmetric = 1 when every edge u->v has an edge v->u (as sets),
+ * else 0.  The frontier modes need it; loaded undirected graphs and the
+ * generators' graphs are symmetric by construction, others are checked
+ * once (O(m log deg) on the host).                                           */
+int glp_graph_symmetric(glp_graph* g, int32_t* symmetric);
 /* Copies the host CSR out (offsets n+1, neighbors m, weights m or NULL).    */
 int glp_graph_csr(const glp_graph* g, int32_t* offsets, int32_t* neighbors,
                   uint32_t* weights);

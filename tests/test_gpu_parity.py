@@ -258,6 +258,22 @@ def test_frontier_gather_equals_dense(scale, tiny, weighted):
         assert int(np.diff(off).max()) > 2048
 
 
+def test_frontier_needs_symmetric_graph():
+    """A changed vertex marks its out-neighbours, which are all the vertices
+    that read its label only when every edge has its reverse: the frontier
+    refuses an asymmetric graph and runs on a symmetric one built from
+    caller arrays (checked, not assumed)."""
+    g = G.Graph.from_csr([0, 1, 2, 2], [1, 2])   # 0->1->2
+    with pytest.raises(G.GLPError):
+        G.run(g, 2, frontier=True)
+    with pytest.raises(G.GLPError):
+        G.run(g, 2, frontier=True, gather=True)
+    off, nbr, _ = G.Graph.rmat(12, 8, seed=61).csr()
+    gs = G.Graph.from_csr(off, nbr)
+    _, dump, _ = G.run(gs, 10, dump=True, frontier=True)
+    assert_steps_equal(dump, O.classic(off, nbr, 10))
+
+
 def test_frontier_rejected_for_layered():
     g = G.Graph.rmat(10, 8, seed=1)
     with pytest.raises(G.GLPError):

@@ -287,3 +287,25 @@ def test_calib_random_read_rejects_bad_arguments(entries, reads):
     with pytest.raises(G.GLPError) as e:
         G.calib_random_read(entries, reads, 1)
     assert e.value.code == -1
+
+
+def test_graph_symmetry(tmp_path):
+    """The frontier marks a changed vertex's out-neighbours, so it needs every
+    edge u->v to have some v->u; loaded undirected graphs and the generators'
+    graphs are symmetric by construction, others are checked (as sets)."""
+    path = G.Graph.from_csr([0, 1, 2, 2], [1, 2])          # 0->1->2
+    assert not path.symmetric()
+    multi = G.Graph.from_csr([0, 2, 3], [1, 1, 0])          # 0->1 twice, 1->0 once
+    assert multi.symmetric()
+    assert not G.Graph.from_csr([0, 2, 3, 4], [1, 2, 0, 1]).symmetric()   # 0->2 without 2->0
+    f = tmp_path / "g.bin"
+    np.array([0, 1, 1, 2, 3, 1], dtype=np.int32).tofile(f)
+    assert G.Graph.load_bin(str(f)).symmetric()
+    assert not G.Graph.load_bin(str(f), directed=True).symmetric()
+    np.array([0, 1, 1, 0], dtype=np.int32).tofile(f)       # both directions listed
+    assert G.Graph.load_bin(str(f), directed=True).symmetric()
+    g = G.Graph.rmat(10, 8, seed=3)
+    assert g.symmetric()
+    off, nbr, _ = g.csr()
+    assert G.Graph.from_csr(off, nbr).symmetric()           # checked, not assumed
+    assert g.degree_sorted()[0].symmetric()
