@@ -124,11 +124,19 @@ struct LayeredHooks {
     __device__ static __forceinline__ uint64_t label_score(const StepArgs& a, int32_t vid,
                                                            uint32_t key, uint32_t freq) {
         (void)vid;
-        // both terms are range-checked on the host to fit int32; with
-        // score_vmul = 0 (gamma = 0) this is classic LP and Vr may be null
-        int32_t s = a.score_mul * (int32_t)freq;
-        if (a.score_vmul) s -= a.score_vmul * a.Vr[key - 1];
-        return pack_score(s, key);
+        return label_score_vol(a, key, freq, label_volume(a, key));
+    }
+    // The score reads V(l) from HBM, so table scans load it for several
+    // labels before scoring them (the loads are then in flight together).
+    static constexpr bool kScoreReadsVolume = true;
+    __device__ static __forceinline__ int32_t label_volume(const StepArgs& a, uint32_t key) {
+        // with score_vmul = 0 (gamma = 0) this is classic LP and Vr may be null
+        return a.score_vmul ? a.Vr[key - 1] : 0;
+    }
+    __device__ static __forceinline__ uint64_t label_score_vol(const StepArgs& a, uint32_t key,
+                                                               uint32_t freq, int32_t vol) {
+        // both terms are range-checked on the host to fit int32
+        return pack_score(a.score_mul * (int32_t)freq - a.score_vmul * vol, key);
     }
     // the write-side volume V_t (header/kernel.cuh:2626 gcounter_temp) is the
     // histogram of the labels written; k_volume computes it after the tiers
@@ -207,6 +215,13 @@ struct count_score_of : std::false_type {};
 template <class H>
 struct count_score_of<H, std::void_t<decltype(H::kScoreIsCount)>>
     : std::bool_constant<H::kScoreIsCount> {};
+
+// Hook sets without kScoreReadsVolume = true score without a per-label load.
+template <class H, class = void>
+struct volume_of : std::false_type {};
+template <class H>
+struct volume_of<H, std::void_t<decltype(H::kScoreReadsVolume)>>
+    : std::bool_constant<H::kScoreReadsVolume> {};
 
 // Hook sets without a kSeeded member have no fixed vertices.
 template <class H, class = void>

@@ -90,6 +90,47 @@ __device__ __forceinline__ void wave_insert(unsigned long long* T, uint32_t mask
     if (ik) kc_insert(T, mask, ik, iff);
 }
 
+// Best packed score over the occupied slots of an LDS table of `cap` slots,
+// `stride` threads cooperating (thread tid takes slots tid, tid + stride,
+// ...); `clear` empties the slots read.  For hooks whose score reads a
+// per-label volume from HBM the volumes of kScanU slots are loaded first,
+// branch-free (an empty slot reads V[0]), so those loads are in flight
+// together instead of one after another.
+constexpr int kScanU = 8;
+template <class H>
+__device__ __forceinline__ uint64_t table_best(const StepArgs& a, int32_t v, unsigned long long* T,
+                                               uint32_t cap, uint32_t tid, uint32_t stride,
+                                               bool clear) {
+    uint64_t best = 0ull;
+    if constexpr (volume_of<H>::value) {
+        for (uint32_t i0 = tid; i0 < cap; i0 += stride * kScanU) {
+            unsigned long long x[kScanU];
+            int32_t vol[kScanU];
+#pragma unroll
+            for (int u = 0; u < kScanU; ++u) {
+                const uint32_t i = i0 + u * stride;
+                x[u] = i < cap ? T[i] : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < kScanU; ++u) vol[u] = H::label_volume(a, x[u] ? (uint32_t)(x[u] >> 32) : 1u);
+#pragma unroll
+            for (int u = 0; u < kScanU; ++u) {
+                if (!x[u]) continue;
+                best = max_u64(best, H::label_score_vol(a, (uint32_t)(x[u] >> 32), (uint32_t)x[u], vol[u]));
+                if (clear) T[i0 + u * stride] = 0ull;
+            }
+        }
+    } else {
+        for (uint32_t i = tid; i < cap; i += stride) {
+            const unsigned long long x = T[i];
+            if (!x) continue;
+            best = max_u64(best, H::label_score(a, v, (uint32_t)(x >> 32), (uint32_t)x));
+            if (clear) T[i] = 0ull;
+        }
+    }
+    return best;
+}
+
 // Counts edges [b, e) of vertex v into an LDS table, STRIDE threads
 // cooperating (thread `tid`; STRIDE a multiple of 64, whole waves).  Each
 // thread first issues the loads of K edges (neighbour ids, then their
@@ -664,11 +705,7 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int4* __restr
                 else if (key[k][i]) kc_insert(T, mask, key[k][i], fr[k][i]);
             }
             wave_lds_sync();
-            for (uint32_t i = lane; i < cap; i += 64) {
-                const unsigned long long x = T[i];
-                if (x) best = max_u64(best, H::label_score(a, v[k], (uint32_t)(x >> 32), (uint32_t)x));
-            }
-            best = wave_max_u64(best);
+            best = wave_max_u64(table_best<H>(a, v[k], T, cap, (uint32_t)lane, 64u, false));
             wave_lds_sync();  // the next vertex clears what this scan read
         }
         bool changed = false;
@@ -714,11 +751,7 @@ __device__ __forceinline__ void block_body(const StepArgs& a, const int32_t* __r
     __syncthreads();
     count_edges<H, W, G, 8, NT>(a, v, b, e, threadIdx.x, T, mask);
     __syncthreads();
-    uint64_t best = 0ull;
-    for (uint32_t i = threadIdx.x; i < cap; i += NT) {
-        const unsigned long long x = T[i];
-        if (x) best = max_u64(best, H::label_score(a, v, (uint32_t)(x >> 32), (uint32_t)x));
-    }
+    uint64_t best = table_best<H>(a, v, T, cap, threadIdx.x, NT, false);
     best = block_max_u64<NT>(best, s_red);
     if (threadIdx.x < 64) {
         bool changed = false;
@@ -1038,13 +1071,7 @@ __device__ __forceinline__ void hub_part_count(const StepArgs& a, const HubArgs&
             hi = lo + max((hi - lo) / 2, (uint64_t)1);
             continue;
         }
-        for (uint32_t i = threadIdx.x; i < cap; i += kHubThreads) {
-            const unsigned long long x = T[i];
-            if (x) {
-                best = max_u64(best, H::label_score(a, j.v, (uint32_t)(x >> 32), (uint32_t)x));
-                T[i] = 0ull;
-            }
-        }
+        best = max_u64(best, table_best<H>(a, j.v, T, cap, threadIdx.x, kHubThreads, true));
         lo = hi;
         hi = end;
     }
