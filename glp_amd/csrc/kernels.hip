@@ -889,6 +889,12 @@ __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h
 // also records the hub's pair total.
 __device__ __forceinline__ void hub_scan_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
     const int32_t v = h.hubs[blk];
+    // a hub the frontier skips this step wrote an all-zero histogram, whose
+    // offsets are all zero already (block-uniform)
+    if (a.act_r && a.fctl[0] == 0 && a.act_r[v] == 0) {
+        if (threadIdx.x == 0) h.hs.total[blk] = 0u;
+        return;
+    }
     const int32_t d = a.off[v + 1] - a.off[v];
     const int64_t nch = (d + h.chunk - 1) / h.chunk;
     const int64_t total = (int64_t)hub_buckets(d, h.part_keys) * nch;
