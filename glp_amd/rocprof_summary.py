@@ -5,6 +5,7 @@
     python -m glp_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR --calibration FILE
             --warmup W --steps K [--gather-steps G] --workload KEY --command "..."
     python -m glp_amd.rocprof_summary sizes DIR --warmup W --steps K [--gather-steps G] --command "..."
+    python -m glp_amd.rocprof_summary steps DIR --steps N --command "..."
 
 `stats` condenses a --kernel-trace --stats run.  `calibrate` compares the
 FETCH_SIZE / WRITE_SIZE that rocprofv3 reports for device-to-device copies of
@@ -12,6 +13,7 @@ a known size with the bytes the copies move and writes the two scale factors.
 `traffic` splits a --pmc run of `bench.py` into LP steps (every step ends
 with one k_end_step dispatch) and reports, per kernel family, the calibrated
 HBM bytes per step over the timed steps; `traffic_for` reads that file back.
+`steps` lists a --kernel-trace run's kernel time per LP step and family.
 """
 import argparse
 import csv
@@ -198,6 +200,39 @@ def traffic_for(path, workload, session, families):
     return tot if found else None
 
 
+def cmd_steps(a):
+    """Per-step kernel time (ms per family) of the first LP session in a
+    --kernel-trace run: dispatches in start order, steps closed by
+    k_end_step, the first `steps` steps."""
+    files = _csvs(a.dir, "kernel_trace.csv")
+    if not files:
+        sys.exit("no kernel_trace.csv under %s" % a.dir)
+    rows = []
+    for f in files:
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                t0, t1 = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+                rows.append((t0, family(r["Kernel_Name"]), (t1 - t0) / 1e6))
+    rows.sort()
+    steps = split_steps([(t, fam, ms) for t, fam, ms in rows])[: a.steps]
+    fams = defaultdict(float)
+    for st in steps:
+        for _, fam, ms in st:
+            if fam != "k_end_step":
+                fams[fam] += ms
+    order = [f for f, _ in sorted(fams.items(), key=lambda kv: -kv[1])][: a.columns]
+    out = ["# kernel time per LP step (ms, from the trace's start/end timestamps)",
+           "# command: %s" % a.command, "#",
+           "step " + " ".join("%13s" % f for f in order) + "        total"]
+    for i, st in enumerate(steps):
+        per = defaultdict(float)
+        for _, fam, ms in st:
+            per[fam] += ms
+        out.append("%4d " % (i + 1) + " ".join("%13.3f" % per.get(f, 0.0) for f in order) +
+                   " %12.3f" % sum(per.values()))
+    print("\n".join(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -224,8 +259,14 @@ def main():
     z.add_argument("--steps", type=int, required=True)
     z.add_argument("--gather-steps", type=int, default=0)
     z.add_argument("--command", required=True)
+    p = sub.add_parser("steps")
+    p.add_argument("dir")
+    p.add_argument("--steps", type=int, required=True, help="steps of the first session to list")
+    p.add_argument("--columns", type=int, default=9, help="kernel families shown (by total time)")
+    p.add_argument("--command", required=True)
     a = ap.parse_args()
-    {"stats": cmd_stats, "calibrate": cmd_calibrate, "traffic": cmd_traffic, "sizes": cmd_sizes}[a.cmd](a)
+    {"stats": cmd_stats, "calibrate": cmd_calibrate, "traffic": cmd_traffic, "sizes": cmd_sizes,
+     "steps": cmd_steps}[a.cmd](a)
 
 
 if __name__ == "__main__":
