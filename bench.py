@@ -375,20 +375,23 @@ def main():
                     "labels_identical_to_dense": same}
         log("frontier %.3f ms/step, labels identical: %s" % (fms, same))
         fs.free()
-        # the separate gather with the frontier: only edges whose source
-        # changed in the previous step are re-gathered once few labels change
-        fg = G.Session(g, G.make_params(method=G.CLASSIC, frontier=True, gather=True))
-        fg.step(args.warmup)
-        fg.sync()
-        t0 = time.perf_counter()
-        fg.step(args.steps)
-        fg.sync()
-        gms_f = (time.perf_counter() - t0) * 1e3 / args.steps
-        same_g = bool(np.array_equal(fg.labels(), dense_labels))
-        frontier["gather_ms_per_step"] = round(gms_f, 4)
-        frontier["gather_labels_identical_to_dense"] = same_g
-        log("frontier gather %.3f ms/step, labels identical: %s" % (gms_f, same_g))
-        fg.free()
+        # the separate gather with the frontier: the labels of sources that
+        # changed in the previous step are pushed to their in-edges once few
+        # change (frontier 2: the two-step lookahead, changes against two
+        # steps back into alternating adj buffers)
+        for mode, key in ((1, "gather"), (2, "gather_lookahead")):
+            fg = G.Session(g, G.make_params(method=G.CLASSIC, frontier=mode, gather=True))
+            fg.step(args.warmup)
+            fg.sync()
+            t0 = time.perf_counter()
+            fg.step(args.steps)
+            fg.sync()
+            gms_f = (time.perf_counter() - t0) * 1e3 / args.steps
+            same_g = bool(np.array_equal(fg.labels(), dense_labels))
+            frontier[key + "_ms_per_step"] = round(gms_f, 4)
+            frontier[key + "_labels_identical_to_dense"] = same_g
+            log("frontier %s %.3f ms/step, labels identical: %s" % (key, gms_f, same_g))
+            fg.free()
 
     # the other methods of the same configurations: layered LP on this graph
     # (gamma 0, 1/4, 1; same warmup and timed steps) and SLPA (T=20, r=0.1)

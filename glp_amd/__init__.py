@@ -236,7 +236,7 @@ def make_params(method=CLASSIC, weighted=False, gamma=(0, 1), seed=1, slpa_T=20,
     p.table_cap = int(table_cap)
     p.gather = int(bool(gather))
     p.profile = int(bool(profile))
-    p.frontier = int(bool(frontier))
+    p.frontier = int(frontier)  # 2: with the gather's two-step lookahead
     return p
 
 

@@ -55,6 +55,10 @@ struct StepArgs {
     // their in-edges when they have few
     const uint32_t* chg_r;
     uint32_t* chg_w;
+    // two-step lookahead of the frontier gather: adj alternates between two
+    // buffers, each refreshed every other step, so the bit records a change
+    // against the label of two steps back (which Lw still holds)
+    int32_t lookahead;
     // seeded LP: seeds[v] != 0 marks a seed (its label stays v)
     const uint8_t* seeds;
 };

@@ -160,6 +160,14 @@ __device__ __forceinline__ void count_edges(const StepArgs& a, int32_t v, int32_
     }
 }
 
+// Whether v sets its bit in the changed-vertex bitmap (frontier gather
+// only): its new label differs from the previous step's, or, with the
+// two-step lookahead, from the label of two steps back, which the buffer
+// this step writes still holds (so this is called before the write).
+__device__ __forceinline__ bool gather_changed(const StepArgs& a, int32_t v, int32_t nl, int32_t old) {
+    if (!a.chg_w) return false;
+    return a.lookahead ? nl != a.Lw[v] : nl != old;
+}
 // Sets v's bit in the changed-vertex bitmap (frontier gather only).
 __device__ __forceinline__ void note_changed(const StepArgs& a, int32_t v) {
     if (a.chg_w) atomicOr(&a.chg_w[v >> 5], 1u << (v & 31));
@@ -384,11 +392,12 @@ __device__ __forceinline__ uint32_t packed_finish(const StepArgs& a, const Packe
         wave_lds_sync();
         best = t.best[lane];
     }
-    bool changed = false;
+    bool changed = false, gch = false;
     if (lane < s.vc && !is_fixed<H>(a, s.v0 + lane)) {
         const int32_t v = s.v0 + lane;
         // no edge, or no counted label: the vertex keeps its label
         const int32_t nl = best ? packed_label(best) : s.old;
+        gch = gather_changed(a, v, nl, s.old);
         H::update_vertex(a, v, nl, best);
         changed = nl != s.old;
     }
@@ -397,9 +406,8 @@ __device__ __forceinline__ uint32_t packed_finish(const StepArgs& a, const Packe
         const bool owner_changed = shfl_i32(changed ? 1 : 0, s.own) != 0;
         if (lane < s.nE && owner_changed) a.act_w[a.nbr[s.base + lane]] = 1;
     }
-    const unsigned long long bal = __ballot(changed);
-    note_changed_run(a, s.v0, bal);
-    return (uint32_t)__popcll(bal);
+    note_changed_run(a, s.v0, __ballot(gch));
+    return (uint32_t)__popcll(__ballot(changed));
 }
 
 // ngroups: groups to run; groups [0, nge) have edges.
@@ -591,9 +599,10 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restr
         bool changed = false;
         if (ql == 0 && run[p]) {
             const int32_t nl = best ? packed_label(best) : old[p];
+            const bool gch = gather_changed(a, v[p], nl, old[p]);
             H::update_vertex(a, v[p], nl, best);
             changed = nl != old[p];
-            if (changed) note_changed(a, v[p]);
+            if (gch) note_changed(a, v[p]);
         }
         nch += (uint32_t)__popcll(__ballot(changed));
         if (marking<H>(a) && shfl_i32(changed ? 1 : 0, g0) != 0)
@@ -711,9 +720,10 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int4* __restr
         bool changed = false;
         if (lane == 0) {
             const int32_t nl = best ? packed_label(best) : old[k];
+            const bool gch = gather_changed(a, v[k], nl, old[k]);
             H::update_vertex(a, v[k], nl, best);
             changed = nl != old[k];
-            if (changed) note_changed(a, v[k]);
+            if (gch) note_changed(a, v[k]);
             nch += changed ? 1u : 0u;
         }
         if (marking<H>(a) && shfl_i32(changed ? 1 : 0, 0) != 0) mark_vertex(a, v[k], b[k], e[k], lane, 64);
@@ -758,9 +768,10 @@ __device__ __forceinline__ void block_body(const StepArgs& a, const int32_t* __r
         if (threadIdx.x == 0) {
             const int32_t old = a.keep[v];
             const int32_t nl = best ? packed_label(best) : old;
+            const bool gch = gather_changed(a, v, nl, old);
             H::update_vertex(a, v, nl, best);
             changed = nl != old;
-            if (changed) note_changed(a, v);
+            if (gch) note_changed(a, v);
             s_changed = changed ? 1u : 0u;
         }
         count_changes(a, changed);
@@ -1129,9 +1140,10 @@ __device__ __forceinline__ void hub_final_body(const StepArgs& a, const HubArgs&
         if (is_active<H>(a, v) && !is_fixed<H>(a, v)) {
             const int32_t old = a.keep[v];
             const int32_t nl = best ? packed_label(best) : old;
+            const bool gch = gather_changed(a, v, nl, old);
             H::update_vertex(a, v, nl, best);
             changed = nl != old;
-            if (changed) note_changed(a, v);
+            if (gch) note_changed(a, v);
         }
         if (marking<H>(a)) {
             h.hs.changed[i] = changed ? 1u : 0u;
@@ -1368,9 +1380,15 @@ __global__ __launch_bounds__(256) void k_gather(const int32_t* __restrict__ nbr,
 // L[u] to the positions of every u that changed in the previous step makes
 // it equal L[nbr]: the work is the changed sources' in-edges, not m.
 //
+// With the two-step lookahead (the reference's onestep = false,
+// tutorial/Lprop.cuh:383-461) adj alternates between two buffers, each
+// refreshed every other step, and a vertex's bit records a change against
+// its label of two steps back: a vertex that flips between two labels then
+// pushes nothing.
+//
 // Frontier control words used here (the rest of fctl is the tiers'):
-//   fctl[3] = 1 once adj has been gathered (set by k_end_step, 0 after a
-//             reset), fctl[4] = push entries listed for this step,
+//   fctl[3] = steps adj has been gathered in, at most 2 (set by k_end_step,
+//             0 after a reset), fctl[4] = push entries listed for this step,
 //   fctl[5] = in-edges of the sources that changed in the previous step,
 //   fctl[6] = 1 when that step changed at most fctl[7] labels (a changed
 //             vertex has an in-edge, so more changes than that rule the push
@@ -1487,14 +1505,17 @@ __device__ __forceinline__ void push_word_counts(const uint32_t* __restrict__ ch
     *cnt = c;
     *edges = e;
 }
-__device__ __forceinline__ bool push_possible(const int32_t* fctl) {
-    return fctl[3] != 0 && fctl[6] != 0;
+// need: steps adj must have been gathered in (1; 2 with the lookahead,
+// whose two buffers are each first filled by a full gather).  The rule-out
+// by the previous step's change count holds for one-step changes only.
+__device__ __forceinline__ bool push_possible(const int32_t* fctl, int32_t need) {
+    return fctl[3] >= need && (need == 2 || fctl[6] != 0);
 }
 __global__ __launch_bounds__(256) void k_push_count(const uint32_t* __restrict__ chg, int64_t nwords,
                                                     const int32_t* __restrict__ inv_off,
                                                     int2* __restrict__ psum,
-                                                    const int32_t* __restrict__ fctl) {
-    if (!push_possible(fctl)) return;  // the gather rewrites every edge this step
+                                                    const int32_t* __restrict__ fctl, int32_t need) {
+    if (!push_possible(fctl, need)) return;  // the gather rewrites every edge this step
     int32_t c, e, tc, te;
     push_word_counts(chg, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, nwords, inv_off, &c, &e);
     (void)block_excl_sum(c, &tc);
@@ -1502,8 +1523,8 @@ __global__ __launch_bounds__(256) void k_push_count(const uint32_t* __restrict__
     if (threadIdx.x == 0) psum[blockIdx.x] = make_int2(tc, te);
 }
 __global__ __launch_bounds__(256) void k_push_scan(int2* __restrict__ psum, int64_t nblocks,
-                                                   int32_t* __restrict__ fctl) {
-    if (!push_possible(fctl)) return;
+                                                   int32_t* __restrict__ fctl, int32_t need) {
+    if (!push_possible(fctl, need)) return;
     int32_t carry = 0, edges = 0;
     for (int64_t b = 0; b < nblocks; b += 256) {
         const int64_t i = b + threadIdx.x;
@@ -1520,16 +1541,16 @@ __global__ __launch_bounds__(256) void k_push_scan(int2* __restrict__ psum, int6
         fctl[5] = edges;
     }
 }
-__device__ __forceinline__ bool push_taken(const int32_t* fctl) {
-    return push_possible(fctl) && fctl[5] <= fctl[7];
+__device__ __forceinline__ bool push_taken(const int32_t* fctl, int32_t need) {
+    return push_possible(fctl, need) && fctl[5] <= fctl[7];
 }
 __global__ __launch_bounds__(256) void k_push_write(const uint32_t* __restrict__ chg, int64_t nwords,
                                                     const int32_t* __restrict__ inv_off,
                                                     const int32_t* __restrict__ L,
                                                     const int2* __restrict__ psum,
                                                     PushEntry* __restrict__ ent,
-                                                    const int32_t* __restrict__ fctl) {
-    if (!push_taken(fctl)) return;
+                                                    const int32_t* __restrict__ fctl, int32_t need) {
+    if (!push_taken(fctl, need)) return;
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int32_t c, e, tc;
     push_word_counts(chg, w, nwords, inv_off, &c, &e);
@@ -1553,8 +1574,8 @@ __global__ __launch_bounds__(256) void k_gather_frontier(const int32_t* __restri
                                                          int32_t* __restrict__ adj, int64_t m,
                                                          const int32_t* __restrict__ inv_pos,
                                                          const PushEntry* __restrict__ ent,
-                                                         const int32_t* __restrict__ fctl) {
-    if (!push_taken(fctl)) {
+                                                         const int32_t* __restrict__ fctl, int32_t need) {
+    if (!push_taken(fctl, need)) {
         gather_all(nbr, L, adj, m);
         return;
     }
@@ -1686,7 +1707,7 @@ __global__ __launch_bounds__(kChangeSlots) void k_end_step(unsigned long long* a
             const int32_t marked = fctl[1];
             fctl[0] = marked ? 0 : 1;
             fctl[1] = sum <= (unsigned long long)fctl[2] ? 1 : 0;
-            fctl[3] = 1;  // a frontier gather has filled adj
+            fctl[3] = min(fctl[3] + 1, 2);  // steps a frontier gather has filled adj in
             fctl[4] = 0;
             fctl[5] = 0;
             fctl[6] = sum <= (unsigned long long)fctl[7] ? 1 : 0;
@@ -1932,14 +1953,15 @@ int launch_frontier_gather(const StepArgs& a, int64_t m, int32_t* adj, const Inv
     auto* psum = static_cast<int2*>(push);
     auto* ent = reinterpret_cast<PushEntry*>(psum + ((nblocks + 1) & ~(int64_t)1));
     int32_t* fctl = const_cast<int32_t*>(a.fctl);
+    const int32_t need = a.lookahead ? 2 : 1;
     hipLaunchKernelGGL(k_push_count, dim3((unsigned)nblocks), dim3(256), 0, s, a.chg_r, nwords, inv.off,
-                       psum, a.fctl);
-    hipLaunchKernelGGL(k_push_scan, dim3(1), dim3(256), 0, s, psum, nblocks, fctl);
+                       psum, a.fctl, need);
+    hipLaunchKernelGGL(k_push_scan, dim3(1), dim3(256), 0, s, psum, nblocks, fctl, need);
     hipLaunchKernelGGL(k_push_write, dim3((unsigned)nblocks), dim3(256), 0, s, a.chg_r, nwords, inv.off,
-                       a.L, psum, ent, a.fctl);
+                       a.L, psum, ent, a.fctl, need);
     const unsigned grid = (unsigned)std::min<int64_t>(grid_for(m / 4 + 1, 256), 256 * 64);
     hipLaunchKernelGGL(k_gather_frontier, dim3(grid), dim3(256), 0, s, a.nbr, a.L, adj, m, inv.pos,
-                       ent, a.fctl);
+                       ent, a.fctl, need);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "gather launch: %s", hipGetErrorString(e));
 }

@@ -20,6 +20,7 @@ struct glp_session {
     int32_t* V[2] = {nullptr, nullptr};  // layered volume histogram, double-buffered
     int32_t* shown = nullptr;            // PickLabel output (SLPA speakers, user hooks)
     int32_t* adj = nullptr;              // pre-gathered neighbour labels (gather mode)
+    int32_t* adj_alt = nullptr;          // the second adj of the two-step lookahead
     int32_t* mem = nullptr;              // SLPA memory (T+1) x n
     void* cm_scratch = nullptr;          // counting-method scratch
     uint8_t* act[2] = {nullptr, nullptr};  // frontier flags, double-buffered like L

@@ -144,6 +144,8 @@ int check_params(const HostCSR& g, const glp_params* p) {
         if (p->reserved[i]) return set_error(GLP_E_ARG, "reserved params fields must be zero");
     if (p->method < GLP_CLASSIC || p->method > GLP_SEEDED)
         return set_error(GLP_E_ARG, "unknown method %d", p->method);
+    if (p->frontier < 0 || p->frontier > 2)
+        return set_error(GLP_E_ARG, "frontier must be 0, 1 or 2 (2: with the gather's two-step lookahead)");
     if (p->frontier && p->method != GLP_CLASSIC)
         return set_error(GLP_E_UNSUPPORTED,
                          "the frontier is exact for classic LP only (layered volumes and SLPA speakers "
@@ -197,6 +199,7 @@ glp_session::~glp_session() {
     }
     (void)hipFree(shown);
     (void)hipFree(adj);
+    (void)hipFree(adj_alt);
     (void)hipFree(mem);
     (void)hipFree(hs.pairs);
     (void)hipFree(hs.sorted);
@@ -235,6 +238,7 @@ size_t session_bytes(const HostCSR& g, const Plan& p, const glp_params& prm) {
     if (prm.gather) b += (size_t)g.m * 4;
     if (prm.frontier) b += 2 * (n + 4);
     if (prm.frontier && prm.gather) b += 2 * ((n + 31) / 32) * 4 + push_entry_bytes(g.n, g.m);
+    if (prm.frontier == 2 && prm.gather) b += (size_t)g.m * 4;  // the lookahead's second adj
     b += (size_t)p.pair_slots * 16 + (size_t)p.hist_slots * 4 + p.hub_chunks.size() * 4 +
          p.hubs.size() * 12;
     b += 8 * kChangeSlots + 8 * kChangeRing + 4;
@@ -277,6 +281,8 @@ int session_create(glp_graph* g, const glp_params* prm, glp_session** out) {
     if (prm->method == GLP_SLPA)
         GLP_HIP(hipMalloc(&s->mem, (size_t)(prm->slpa_T + 1) * nn * 4));
     if (prm->gather) GLP_HIP(hipMalloc(&s->adj, std::max<size_t>((size_t)g->csr.m, 1) * 4));
+    if (prm->gather && prm->frontier == 2)
+        GLP_HIP(hipMalloc(&s->adj_alt, std::max<size_t>((size_t)g->csr.m, 1) * 4));
     if (!p.hubs.empty()) {
         GLP_HIP(hipMalloc(&s->hs.pairs, (size_t)p.pair_slots * 8));
         GLP_HIP(hipMalloc(&s->hs.sorted, (size_t)p.pair_slots * 8));
@@ -415,7 +421,9 @@ static StepArgs step_args(glp_session* s) {
             a.Lw = s->mem + (size_t)(s->t + 1) * s->n;
             break;
     }
-    a.adj = s->adj;
+    // the lookahead's two adj buffers alternate with the label buffers, so
+    // each captured step graph keeps its own pair
+    a.adj = s->adj_alt && c ? s->adj_alt : s->adj;
     if (s->fctl) {
         a.act_r = s->act[c];
         a.act_w = s->act[c ^ 1];
@@ -424,6 +432,7 @@ static StepArgs step_args(glp_session* s) {
     if (s->chg[0]) {
         a.chg_r = s->chg[c];
         a.chg_w = s->chg[c ^ 1];
+        a.lookahead = s->adj_alt ? 1 : 0;
     }
     return a;
 }
@@ -450,8 +459,9 @@ int enqueue_step(glp_session* s, hipEvent_t* ev) {
     if (ev) (void)hipEventRecord(ev[2 * GLP_TIER_PICK + 1], s->stream);
     if (ev) (void)hipEventRecord(ev[2 * GLP_TIER_GATHER], s->stream);
     if (s->p.gather) {
-        if (s->chg[0]) rc = launch_frontier_gather(a, s->g->csr.m, s->adj, s->g->dev->inv, s->push, s->stream);
-        else rc = launch_gather(a, s->g->csr.m, s->adj, s->stream);
+        int32_t* adj = const_cast<int32_t*>(a.adj);
+        if (s->chg[0]) rc = launch_frontier_gather(a, s->g->csr.m, adj, s->g->dev->inv, s->push, s->stream);
+        else rc = launch_gather(a, s->g->csr.m, adj, s->stream);
         if (rc) return rc;
     }
     if (ev) (void)hipEventRecord(ev[2 * GLP_TIER_GATHER + 1], s->stream);

@@ -71,7 +71,11 @@ typedef struct glp_params {
     int32_t frontier;     /* classic only: 1 = recompute only vertices with a
                              changed neighbour once few labels change (the
                              reference's if_update frontier,
-                             header/kernel.cuh:21-35); labels are identical  */
+                             header/kernel.cuh:21-35); labels are identical.
+                             2 = the same, and with gather = 1 the gather's
+                             two-step lookahead (tutorial/Lprop.cuh:383-461:
+                             two adj buffers, changes against two steps
+                             back); needs a symmetric graph              */
     int32_t reserved[5];  /* must be zero                                        */
 } glp_params;
 

@@ -217,44 +217,49 @@ def test_frontier_equals_dense(scale, tiny):
     assert ch[-1] < (len(off) - 1) // 32   # the frontier was in use at the end
 
 
-def _push_steps(off, nbr, want):
-    """Steps (1-based) whose frontier gather takes the push form: from step 2
-    on, when the sources that changed in the step before have at most m/8
-    in-edges (the gather's own rule)."""
+def _push_steps(off, nbr, want, back=1):
+    """Steps (1-based) whose frontier gather takes the push form: once adj
+    has been gathered `back` times, when the sources whose label in the step
+    before differs from their label `back` steps earlier have at most m/8
+    in-edges (the gather's own rule; back = 2 is the two-step lookahead)."""
     n, m = len(off) - 1, len(nbr)
     indeg = np.bincount(nbr, minlength=n)
     labs = [np.arange(n, dtype=np.int32)] + list(want)
-    return [t for t in range(2, len(labs))
-            if int(indeg[labs[t - 1] != labs[t - 2]].sum()) <= m // 8]
+    return [t for t in range(back + 1, len(labs))
+            if int(indeg[labs[t - 1] != labs[t - 1 - back]].sum()) <= m // 8]
 
 
-@pytest.mark.parametrize("scale,tiny,weighted", [(14, False, False), (13, True, False),
-                                                 (13, False, True), (16, False, False)])
-def test_frontier_gather_equals_dense(scale, tiny, weighted):
+@pytest.mark.parametrize("scale,tiny,weighted,frontier", [(14, False, False, 1), (13, True, False, 1),
+                                                          (13, False, True, 1), (16, False, False, 1),
+                                                          (14, False, False, 2), (13, True, False, 2),
+                                                          (13, False, True, 2), (16, False, False, 2)])
+def test_frontier_gather_equals_dense(scale, tiny, weighted, frontier):
     """Separate gather with the frontier: after a step whose changed sources
     have few in-edges the gather pushes their labels through the inverse
     index, otherwise it rewrites every edge; every step must equal the
     oracle, also under graph replay and across a checkpoint (which forces a
     full gather).  Sources with more in-edges than one push entry holds
-    (256) occur at every scale here; at scale 16 some have thousands."""
+    (256) occur at every scale here; at scale 16 some have thousands.
+    frontier=2 adds the two-step lookahead (two adj buffers, changes
+    against two steps back)."""
     kw = dict(pack_max_deg=8, wave_max_deg=16, block_max_deg=32, hub_chunk=256,
               part_keys=256) if tiny else {}
     with G.tuning(**kw):
         g = G.Graph.rmat(scale, 16, seed=70 + scale, wmax=8 if weighted else 0)
         off, nbr, w = g.csr()
         want = O.classic(off, nbr, 16, w=w if weighted else None)
-        _, dump, _ = G.run(g, 16, dump=True, frontier=True, gather=True, weighted=weighted)
+        _, dump, _ = G.run(g, 16, dump=True, frontier=frontier, gather=True, weighted=weighted)
         assert_steps_equal(dump, want)
-        with G.Session(g, frontier=True, gather=True, weighted=weighted) as s:
+        with G.Session(g, frontier=frontier, gather=True, weighted=weighted) as s:
             s.step(3)
             s.step(8, graph=True)
             assert np.array_equal(s.labels(), want[10])
             state = s.save()
-        with G.Session(g, frontier=True, gather=True, weighted=weighted) as s2:
+        with G.Session(g, frontier=frontier, gather=True, weighted=weighted) as s2:
             s2.load(state)
             s2.step(5)
             assert np.array_equal(s2.labels(), want[15])
-    push = _push_steps(off, nbr, want)
+    push = _push_steps(off, nbr, want, back=frontier)
     assert len(push) >= 4 and push[-1] == 16, push   # the push form was in use
     assert int(np.diff(off).max()) > 256
 
