@@ -491,6 +491,9 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restr
             if (!run[p] || j >= e[p]) key[p][i] = 0u;
         }
     unsigned long long* T = s_half[wv].slot[q];
+    // the tables are cleared before their first use, and each unit leaves
+    // them empty: a slot's creator clears it once it has scored it
+    bool cleared = false;  // wave-uniform
     // lanes of this group where x holds
     auto group_count = [&](bool x) { return (uint32_t)__popcll(__ballot(x) & gm); };
     // sum over this group's lanes (xor distances below S stay in the group)
@@ -552,10 +555,12 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restr
         }
         uint64_t best = fast;
         if (__ballot(run[p] && fast == 0ull) != 0ull) {  // wave-uniform: some group needs the table
-            wave_lds_sync();  // the previous unit's reads are done
+            if (!cleared) {
 #pragma unroll
-            for (uint32_t i = 0; i < kHalfCap; i += S) T[i + ql] = 0ull;
-            wave_lds_sync();
+                for (uint32_t i = 0; i < kHalfCap; i += S) T[i + ql] = 0ull;
+                cleared = true;
+            }
+            wave_lds_sync();  // the clear, or the previous unit's clears, are done
             // per group, the lanes holding the group's first pending label
             // merge into one insert by the lowest of them (a converged vertex
             // shows one label on most edges, and same-address LDS atomics
@@ -589,8 +594,11 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restr
             uint64_t pm = 0ull;
 #pragma unroll
             for (int i = 0; i < EPL; ++i)
-                if (sk[i] && (hs[i] & kClaimed))
-                    pm = max_u64(pm, H::label_score(a, v[p], sk[i], (uint32_t)T[hs[i] & ~kClaimed]));
+                if (sk[i] && (hs[i] & kClaimed)) {
+                    unsigned long long* slot = &T[hs[i] & ~kClaimed];
+                    pm = max_u64(pm, H::label_score(a, v[p], sk[i], (uint32_t)*slot));
+                    *slot = 0ull;
+                }
             // max over the group (xor distances below S stay in the group)
 #pragma unroll
             for (int d = S / 2; d >= 1; d >>= 1) pm = max_u64(pm, shfl_u64(pm, lane ^ d));
