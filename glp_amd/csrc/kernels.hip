@@ -7,8 +7,12 @@
 // :1655-1739 (big), :3124-3222 (medium), :4365-4426 (small), :5187-5234
 // (tiny)).  The work is split by degree:
 //   packed  deg <= pack_max   several vertices share one wavefront: <= 64
-//                             edges, counted by an all-pairs compare inside
-//                             each vertex's lane segment, segmented max
+//                             edges, a two-candidate majority test per
+//                             vertex, else an (owner, label) LDS table and
+//                             a per-owner max
+//   half    deg <= half_max   16 or 32 lanes per vertex (<= 64 edges), the
+//                             same majority test, else an LDS table per lane
+//                             group
 //   wave    deg <= wave_max   one wavefront per vertex, LDS hash table
 //   block   deg <= block_max  one workgroup per vertex, LDS hash table sized
 //                             by the vertex's degree class
