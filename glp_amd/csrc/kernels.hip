@@ -908,6 +908,36 @@ __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h
     if (threadIdx.x == 0) h.hs.chunk_cnt[blk] = s_npairs;
 }
 
+// Sum of hist[lo, hi), R entries per round loaded branch-free (clamped).
+template <int R>
+__device__ __forceinline__ uint32_t hist_run_sum(const uint32_t* hist, int64_t lo, int64_t hi) {
+    uint32_t sum = 0u;
+    for (int64_t i = lo; i < hi; i += R) {
+        uint32_t x[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) x[k] = hist[min(i + k, hi - 1)];
+#pragma unroll
+        for (int k = 0; k < R; ++k) sum += i + k < hi ? x[k] : 0u;
+    }
+    return sum;
+}
+// Replaces hist[lo, hi) by its exclusive prefix sums starting at `run`.
+template <int R>
+__device__ __forceinline__ void hist_run_offsets(uint32_t* hist, int64_t lo, int64_t hi, uint32_t run) {
+    for (int64_t i = lo; i < hi; i += R) {
+        uint32_t x[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) x[k] = hist[min(i + k, hi - 1)];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (i + k < hi) {
+                hist[i + k] = run;
+                run += x[k];
+            }
+        }
+    }
+}
+
 // In-place exclusive scan of one hub's P x nch histogram (bucket-major);
 // also records the hub's pair total.
 __device__ __forceinline__ void hub_scan_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
@@ -926,32 +956,19 @@ __device__ __forceinline__ void hub_scan_body(const StepArgs& a, const HubArgs& 
     const int64_t per = (total + kHubThreads - 1) / kHubThreads;
     const int64_t lo = min((int64_t)threadIdx.x * per, total);
     const int64_t hi = min(lo + per, total);
-    // kR entries per round, loaded branch-free (clamped) so they are in
-    // flight together: the largest hubs' runs are hundreds of entries long
-    constexpr int kR = 64;
-    uint32_t sum = 0u;
-    for (int64_t i = lo; i < hi; i += kR) {
-        uint32_t x[kR];
-#pragma unroll
-        for (int k = 0; k < kR; ++k) x[k] = hist[min(i + k, hi - 1)];
-#pragma unroll
-        for (int k = 0; k < kR; ++k) sum += i + k < hi ? x[k] : 0u;
-    }
+    // R entries per round, loaded branch-free (clamped) so they are in
+    // flight together: the largest hubs' runs are hundreds of entries long,
+    // most hubs' one or two (per is block-uniform)
+    uint32_t sum;
+    if (per <= 4) sum = hist_run_sum<4>(hist, lo, hi);
+    else if (per <= 16) sum = hist_run_sum<16>(hist, lo, hi);
+    else sum = hist_run_sum<64>(hist, lo, hi);
     int32_t pairs;
-    uint32_t run = (uint32_t)block_excl_sum((int32_t)sum, &pairs);
+    const uint32_t run = (uint32_t)block_excl_sum((int32_t)sum, &pairs);
     if (threadIdx.x == 0) h.hs.total[blk] = (uint32_t)pairs;
-    for (int64_t i = lo; i < hi; i += kR) {
-        uint32_t x[kR];
-#pragma unroll
-        for (int k = 0; k < kR; ++k) x[k] = hist[min(i + k, hi - 1)];
-#pragma unroll
-        for (int k = 0; k < kR; ++k) {
-            if (i + k < hi) {
-                hist[i + k] = run;
-                run += x[k];
-            }
-        }
-    }
+    if (per <= 4) hist_run_offsets<4>(hist, lo, hi, run);
+    else if (per <= 16) hist_run_offsets<16>(hist, lo, hi, run);
+    else hist_run_offsets<64>(hist, lo, hi, run);
 }
 
 // Moves one chunk's pairs to their bucket positions in the hub's sorted area.
