@@ -330,7 +330,8 @@ __device__ __forceinline__ bool packed_load_safe(const StepArgs& a, int2 gr, Pac
 // Counts, picks and writes the group's labels (all lanes active); returns
 // how many of them changed.
 template <class H, bool W, bool G>
-__device__ __forceinline__ uint32_t packed_finish(const StepArgs& a, const PackedGroup& s, PackedLds& t) {
+__device__ __forceinline__ uint32_t packed_finish(const StepArgs& a, const PackedGroup& s, PackedLds& t,
+                                                  bool& cleared) {
     const int lane = lane_id();
     // key 0 is an unlabeled neighbour (seeded LP): it never counts
     const bool live = lane < s.nE && s.key != 0u;
@@ -361,13 +362,18 @@ __device__ __forceinline__ uint32_t packed_finish(const StepArgs& a, const Packe
         decided = __ballot(open) == 0ull;  // wave-uniform
     }
     if (!decided) {
-        wave_lds_sync();  // the previous group's reads of the table are done
-        t.key[lane] = 0ull;
-        t.key[lane + 64] = 0ull;
-        t.cnt[lane] = 0u;
-        t.cnt[lane + 64] = 0u;
-        t.best[lane] = 0ull;
-        wave_lds_sync();
+        // the table is cleared before its first use in the wave, and each
+        // group leaves it empty: a slot's creator clears its slot once it
+        // has scored it, and each vertex lane its best word once read
+        if (!cleared) {  // wave-uniform
+            t.key[lane] = 0ull;
+            t.key[lane + 64] = 0ull;
+            t.cnt[lane] = 0u;
+            t.cnt[lane + 64] = 0u;
+            t.best[lane] = 0ull;
+            cleared = true;
+        }
+        wave_lds_sync();  // the clear, or the previous group's clears, are done
         uint32_t h = 0u;
         bool claimed = false;  // this lane created the (owner, label) slot
         if (live) {
@@ -392,9 +398,12 @@ __device__ __forceinline__ uint32_t packed_finish(const StepArgs& a, const Packe
         if (claimed) {
             const uint64_t p = H::label_score(a, s.v0 + s.own, s.key, t.cnt[h]);
             atomicMax(&t.best[s.own], (unsigned long long)p);
+            t.key[h] = 0ull;
+            t.cnt[h] = 0u;
         }
         wave_lds_sync();
         best = t.best[lane];
+        t.best[lane] = 0ull;
     }
     bool changed = false, gch = false;
     if (lane < s.vc && !is_fixed<H>(a, s.v0 + lane)) {
@@ -424,17 +433,18 @@ __device__ __forceinline__ void packed_body(const StepArgs& a, const int2* __res
         __builtin_amdgcn_readfirstlane((int32_t)((blk * 256u + threadIdx.x) >> 6)) * NG;
     if (g0 >= ngroups) return;
     PackedLds& t = s_pk[threadIdx.x >> 6];
+    bool cleared = false;  // the wave's table (wave-uniform)
     uint32_t nch = 0u;
     if (g0 + NG <= nge && !frontier_on<H>(a)) {
         PackedGroup s[NG];
 #pragma unroll
         for (int g = 0; g < NG; ++g) packed_load_fast<H, W, G>(a, groups[g0 + g], s[g]);
 #pragma unroll
-        for (int g = 0; g < NG; ++g) nch += packed_finish<H, W, G>(a, s[g], t);
+        for (int g = 0; g < NG; ++g) nch += packed_finish<H, W, G>(a, s[g], t, cleared);
     } else {
         for (int g = 0; g < NG && g0 + g < ngroups; ++g) {
             PackedGroup s;
-            if (packed_load_safe<H, W, G>(a, groups[g0 + g], s)) nch += packed_finish<H, W, G>(a, s, t);
+            if (packed_load_safe<H, W, G>(a, groups[g0 + g], s)) nch += packed_finish<H, W, G>(a, s, t, cleared);
         }
     }
     add_changes(a, nch);
