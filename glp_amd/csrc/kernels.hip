@@ -767,7 +767,7 @@ __device__ __forceinline__ uint64_t block_max_u64(uint64_t x, unsigned long long
     return x;  // valid in wave 0
 }
 
-template <class H, bool W, bool G, int NT>
+template <class H, bool W, bool G, int NT, int K = 8>
 __device__ __forceinline__ void block_body(const StepArgs& a, const int32_t* __restrict__ verts,
                                            uint32_t cap_max, uint32_t blk) {
     extern __shared__ __align__(16) uint32_t s_dyn[];
@@ -781,7 +781,7 @@ __device__ __forceinline__ void block_body(const StepArgs& a, const int32_t* __r
     const uint32_t mask = cap - 1u;
     for (uint32_t i = threadIdx.x; i < cap; i += NT) T[i] = 0ull;
     __syncthreads();
-    count_edges<H, W, G, 8, NT>(a, v, b, e, threadIdx.x, T, mask);
+    count_edges<H, W, G, K, NT>(a, v, b, e, threadIdx.x, T, mask);
     __syncthreads();
     uint64_t best = table_best<H>(a, v, T, cap, threadIdx.x, NT, false);
     best = block_max_u64<NT>(best, s_red);
@@ -1222,10 +1222,10 @@ __global__ __launch_bounds__(256) void k_wave(StepArgs a, const int4* __restrict
                                               int32_t nv) {
     wave_body<H, W, G>(a, verts, nv, blockIdx.x);
 }
-template <class H, bool W, bool G, int NT>
+template <class H, bool W, bool G, int NT, int K = 8>
 __global__ __launch_bounds__(NT) void k_block(StepArgs a, const int32_t* __restrict__ verts,
                                               uint32_t cap_max) {
-    block_body<H, W, G, NT>(a, verts, cap_max, blockIdx.x);
+    block_body<H, W, G, NT, K>(a, verts, cap_max, blockIdx.x);
 }
 template <class H, bool W, bool G>
 __global__ __launch_bounds__(kHubThreads) void k_hub_agg(StepArgs a, HubArgs h) {
@@ -1844,6 +1844,14 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
         } else if (p.tune.block_threads == 1024) {
             hipLaunchKernelGGL((k_block<H, W, G, 1024>), dim3(nb), dim3(1024), lds, s, a,
                                g.d_block[c], cap);
+        } else if (cap <= 1024) {
+            // gathers in flight per thread sized to the class's largest
+            // degree (cap / 2 edges over 256 threads)
+            hipLaunchKernelGGL((k_block<H, W, G, 256, 2>), dim3(nb), dim3(256), lds, s, a,
+                               g.d_block[c], cap);
+        } else if (cap <= 2048) {
+            hipLaunchKernelGGL((k_block<H, W, G, 256, 4>), dim3(nb), dim3(256), lds, s, a,
+                               g.d_block[c], cap);
         } else {
             hipLaunchKernelGGL((k_block<H, W, G, 256>), dim3(nb), dim3(256), lds, s, a,
                                g.d_block[c], cap);
@@ -1908,6 +1916,10 @@ void init_kernels() {
     const int lim = 160 * 1024 - 1024;
 #define GLP_SETLDS(HK, W, G)                                                                    \
     (void)hipFuncSetAttribute((const void*)k_block<HK, W, G, 256>,                              \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim);                 \
+    (void)hipFuncSetAttribute((const void*)k_block<HK, W, G, 256, 2>,                           \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim);                 \
+    (void)hipFuncSetAttribute((const void*)k_block<HK, W, G, 256, 4>,                           \
                               hipFuncAttributeMaxDynamicSharedMemorySize, lim);                 \
     (void)hipFuncSetAttribute((const void*)k_block<HK, W, G, 512>,                              \
                               hipFuncAttributeMaxDynamicSharedMemorySize, lim);                 \
