@@ -47,7 +47,8 @@ struct Plan {
     // wave-tier vertices as {vertex, first edge, end of edges, 0}: the edge
     // range travels with the vertex, so a wave's loads skip the offsets
     std::vector<int4> half;                   // deg <= half_max_deg (half-wave each)
-    std::vector<int4> wave;                   // larger degrees
+    std::vector<int4> wave;                   // larger degrees: <= 128 first, then the rest
+    int64_t wave_short = 0;                   // wave entries of degree <= 128 (the first ones)
     int32_t nclasses = 0;
     int32_t class_cap[kMaxBlockClasses] = {};  // LDS slots of each block class
     std::vector<int32_t> block[kMaxBlockClasses];

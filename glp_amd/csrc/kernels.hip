@@ -639,7 +639,9 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restr
 // kWaveNV vertices are issued first, then each vertex is counted in the
 // wave's LDS table in turn.
 constexpr int kWaveNV = 4;
-template <class H, bool W, bool G>
+// EPL: edges loaded per lane (4 covers 256 edges; 2 serves vertices of at
+// most 128 edges without clamped duplicate loads)
+template <class H, bool W, bool G, int EPL = 4>
 __device__ __forceinline__ void wave_body(const StepArgs& a, const int4* __restrict__ verts,
                                           int32_t nv, uint32_t blk) {
     __shared__ unsigned long long s_tab[4][kWaveCap];  // (label key << 32 | count) slots
@@ -648,7 +650,7 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int4* __restr
     if (i0 >= nv) return;  // wave-uniform
     int32_t v[kWaveNV], b[kWaveNV], e[kWaveNV], old[kWaveNV];
     bool run[kWaveNV];
-    uint32_t key[kWaveNV][4], fr[kWaveNV][4];
+    uint32_t key[kWaveNV][EPL], fr[kWaveNV][EPL];
 #pragma unroll
     for (int k = 0; k < kWaveNV; ++k) {
         const int4 ent = verts[min(i0 + k, nv - 1)];
@@ -666,7 +668,7 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int4* __restr
 #pragma unroll
     for (int k = 0; k < kWaveNV; ++k)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < EPL; ++i) {
             const int32_t j = b[k] + lane + 64 * i;
             key[k][i] = H::template load_neighbor<W, G>(a, v[k], run[k] ? min(j, e[k] - 1) : b[k], &fr[k][i]);
             if (j >= e[k]) key[k][i] = 0u;
@@ -688,7 +690,7 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int4* __restr
             uint32_t c = 0u, c2 = 0u, tot = 0u;
             if (W) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
+                for (int i = 0; i < EPL; ++i) {
                     c += key[k][i] && key[k][i] == lead ? fr[k][i] : 0u;
                     c2 += key[k][i] && key[k][i] == lead2 ? fr[k][i] : 0u;
                     tot += key[k][i] ? fr[k][i] : 0u;
@@ -701,16 +703,16 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int4* __restr
                 // every edge of the vertex counts once
                 if constexpr (seeded_of<H>::value) {
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) tot += (uint32_t)__popcll(__ballot(key[k][i] != 0u));
+                    for (int i = 0; i < EPL; ++i) tot += (uint32_t)__popcll(__ballot(key[k][i] != 0u));
                 } else {
                     tot = (uint32_t)(e[k] - b[k]);
                 }
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
+                for (int i = 0; i < EPL; ++i)
                     c += (uint32_t)__popcll(__ballot(key[k][i] != 0u && key[k][i] == lead));
                 if (!(lead != 0u && 2ull * c > (unsigned long long)tot)) {  // wave-uniform
 #pragma unroll
-                    for (int i = 0; i < 4; ++i)
+                    for (int i = 0; i < EPL; ++i)
                         c2 += (uint32_t)__popcll(__ballot(key[k][i] != 0u && key[k][i] == lead2));
                 }
             }
@@ -731,7 +733,7 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int4* __restr
             for (uint32_t i = lane; i < cap; i += 64) T[i] = 0ull;
             wave_lds_sync();
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < EPL; ++i) {
                 if (peel) wave_insert<W>(T, mask, key[k][i], fr[k][i]);
                 else if (key[k][i]) kc_insert(T, mask, key[k][i], fr[k][i]);
             }
@@ -1217,10 +1219,10 @@ __global__ __launch_bounds__(256) void k_half(StepArgs a, const int4* __restrict
                                               int32_t nv) {
     half_body<H, W, G, NP, S>(a, verts, nv, blockIdx.x);
 }
-template <class H, bool W, bool G>
+template <class H, bool W, bool G, int EPL>
 __global__ __launch_bounds__(256) void k_wave(StepArgs a, const int4* __restrict__ verts,
                                               int32_t nv) {
-    wave_body<H, W, G>(a, verts, nv, blockIdx.x);
+    wave_body<H, W, G, EPL>(a, verts, nv, blockIdx.x);
 }
 template <class H, bool W, bool G, int NT, int K = 8>
 __global__ __launch_bounds__(NT) void k_block(StepArgs a, const int32_t* __restrict__ verts,
@@ -1827,9 +1829,15 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
         else launch_half<H, W, G, 32>(a, g.d_half, nh, s);
     }
     if (!p.wave.empty()) {
-        const int32_t nw = (int32_t)p.wave.size();
-        hipLaunchKernelGGL((k_wave<H, W, G>), dim3(grid_for(nw, 4 * kWaveNV)), dim3(256), 0, s, a,
-                           g.d_wave, nw);
+        // degrees <= 128 (the list's first wave_short entries) load two edges
+        // per lane, the rest four
+        const int32_t ns = (int32_t)p.wave_short, nw = (int32_t)p.wave.size();
+        if (ns > 0)
+            hipLaunchKernelGGL((k_wave<H, W, G, 2>), dim3(grid_for(ns, 4 * kWaveNV)), dim3(256), 0, s, a,
+                               g.d_wave, ns);
+        if (nw > ns)
+            hipLaunchKernelGGL((k_wave<H, W, G, 4>), dim3(grid_for(nw - ns, 4 * kWaveNV)), dim3(256), 0,
+                               s, a, g.d_wave + ns, nw - ns);
     }
     mark(GLP_TIER_WAVE, 1);
     mark(GLP_TIER_BLOCK, 0);

@@ -182,6 +182,15 @@ int build_plan(const HostCSR& g, const Tuning& t, Plan& p) {
         p.groups = std::move(with_edges);
         p.groups.insert(p.groups.end(), isolated.begin(), isolated.end());
     }
+    // wave-tier vertices of degree <= 128 first: their waves load two edges
+    // per lane instead of four
+    {
+        std::vector<int4> shrt, rest;
+        for (const int4& ent : p.wave) (ent.z - ent.y <= 128 ? shrt : rest).push_back(ent);
+        p.wave_short = (int64_t)shrt.size();
+        p.wave = std::move(shrt);
+        p.wave.insert(p.wave.end(), rest.begin(), rest.end());
+    }
     if (hub_chunks > INT32_MAX) return set_error(GLP_E_RANGE, "too many hub chunks");
     return GLP_OK;
 }
