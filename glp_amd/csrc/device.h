@@ -46,7 +46,8 @@ struct Plan {
     int64_t groups_with_edges = 0;            // groups before the edge-less ones
     // wave-tier vertices as {vertex, first edge, end of edges, 0}: the edge
     // range travels with the vertex, so a wave's loads skip the offsets
-    std::vector<int4> half;                   // deg <= half_max_deg (half-wave each)
+    std::vector<int4> half;                   // deg <= half_max_deg: <= 32 first, then the rest
+    int64_t half_short = 0;                   // half entries of degree <= 32 (the first ones)
     std::vector<int4> wave;                   // larger degrees: <= 128 first, then the rest
     int64_t wave_short = 0;                   // wave entries of degree <= 128 (the first ones)
     int32_t nclasses = 0;

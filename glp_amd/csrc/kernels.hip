@@ -465,12 +465,12 @@ struct HalfLds {
     unsigned long long slot[64 / S][kHalfCap];
 };
 
-template <class H, bool W, bool G, int NP, int S = 32>
+// EPL: edges loaded per lane (S * EPL >= the launch's largest degree)
+template <class H, bool W, bool G, int NP, int S = 32, int EPL = kHalfMaxDeg / S>
 __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restrict__ verts,
                                           int32_t nv, uint32_t blk) {
     static_assert(S == 16 || S == 32, "16 or 32 lanes per vertex");
     constexpr int V = 64 / S;               // vertices per unit
-    constexpr int EPL = kHalfMaxDeg / S;    // edges per lane
     __shared__ HalfLds<S> s_half[4];
     const int lane = lane_id(), wv = threadIdx.x >> 6, q = lane / S, ql = lane % S;
     const int g0 = q * S;  // the group's first lane
@@ -1214,10 +1214,10 @@ __global__ __launch_bounds__(256) void k_packed(StepArgs a, const int2* __restri
                                                 int32_t ngroups, int32_t nge) {
     packed_body<H, W, G, NG>(a, groups, ngroups, nge, blockIdx.x);
 }
-template <class H, bool W, bool G, int NP, int S>
+template <class H, bool W, bool G, int NP, int S, int EPL>
 __global__ __launch_bounds__(256) void k_half(StepArgs a, const int4* __restrict__ verts,
                                               int32_t nv) {
-    half_body<H, W, G, NP, S>(a, verts, nv, blockIdx.x);
+    half_body<H, W, G, NP, S, EPL>(a, verts, nv, blockIdx.x);
 }
 template <class H, bool W, bool G, int EPL>
 __global__ __launch_bounds__(256) void k_wave(StepArgs a, const int4* __restrict__ verts,
@@ -1773,21 +1773,22 @@ static inline unsigned grid_for(int64_t work, int per_block) {
 
 // The half tier with S lanes per vertex: a unit is 64/S vertices, and a
 // wave takes 4, 2 or 1 units (units_per_wave).
-template <class H, bool W, bool G, int S>
+template <class H, bool W, bool G, int S, int EPL = kHalfMaxDeg / S>
 static void launch_half(const StepArgs& a, const int4* verts, int32_t nh, hipStream_t s) {
     constexpr int V = 64 / S;
+    if (nh <= 0) return;
     switch (units_per_wave((nh + V - 1) / V)) {
         case 4:
-            hipLaunchKernelGGL((k_half<H, W, G, 4, S>), dim3(grid_for(nh, 16 * V)), dim3(256), 0, s, a,
-                               verts, nh);
+            hipLaunchKernelGGL((k_half<H, W, G, 4, S, EPL>), dim3(grid_for(nh, 16 * V)), dim3(256), 0, s,
+                               a, verts, nh);
             break;
         case 2:
-            hipLaunchKernelGGL((k_half<H, W, G, 2, S>), dim3(grid_for(nh, 8 * V)), dim3(256), 0, s, a,
-                               verts, nh);
+            hipLaunchKernelGGL((k_half<H, W, G, 2, S, EPL>), dim3(grid_for(nh, 8 * V)), dim3(256), 0, s,
+                               a, verts, nh);
             break;
         default:
-            hipLaunchKernelGGL((k_half<H, W, G, 1, S>), dim3(grid_for(nh, 4 * V)), dim3(256), 0, s, a,
-                               verts, nh);
+            hipLaunchKernelGGL((k_half<H, W, G, 1, S, EPL>), dim3(grid_for(nh, 4 * V)), dim3(256), 0, s,
+                               a, verts, nh);
     }
 }
 
@@ -1825,8 +1826,14 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
         // enough vertices to fill the chip with waves of four units
         const bool quarter = p.tune.half_lanes == 16 ||
                              (p.tune.half_lanes == 0 && nh / 16 >= kFillWaves);
-        if (quarter) launch_half<H, W, G, 16>(a, g.d_half, nh, s);
-        else launch_half<H, W, G, 32>(a, g.d_half, nh, s);
+        if (quarter) {
+            // with 16 lanes, degrees <= 32 (listed first) take two edges per lane
+            const int32_t ns = (int32_t)p.half_short;
+            launch_half<H, W, G, 16, 2>(a, g.d_half, ns, s);
+            launch_half<H, W, G, 16, 4>(a, g.d_half + ns, nh - ns, s);
+        } else {
+            launch_half<H, W, G, 32>(a, g.d_half, nh, s);
+        }
     }
     if (!p.wave.empty()) {
         // degrees <= 128 (the list's first wave_short entries) load two edges

@@ -182,6 +182,15 @@ int build_plan(const HostCSR& g, const Tuning& t, Plan& p) {
         p.groups = std::move(with_edges);
         p.groups.insert(p.groups.end(), isolated.begin(), isolated.end());
     }
+    // half-tier vertices of degree <= 32 first: with 16 lanes per vertex
+    // they load two edges per lane instead of four
+    {
+        std::vector<int4> shrt, rest;
+        for (const int4& ent : p.half) (ent.z - ent.y <= 32 ? shrt : rest).push_back(ent);
+        p.half_short = (int64_t)shrt.size();
+        p.half = std::move(shrt);
+        p.half.insert(p.half.end(), rest.begin(), rest.end());
+    }
     // wave-tier vertices of degree <= 128 first: their waves load two edges
     // per lane instead of four
     {
