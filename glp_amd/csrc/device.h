@@ -61,6 +61,7 @@ struct Plan {
     std::vector<int64_t> hub_hist_base;       // first histogram entry of each hub
     int64_t pair_slots = 0;                   // sum over hubs of chunks * hub_chunk
     int64_t hist_slots = 0;                   // sum over hubs of buckets * chunks
+    int32_t hub_max_buckets = 0;              // most buckets of any hub (sizes k_hub_agg's LDS)
     int64_t edges[4] = {}, verts[4] = {};     // per tier: packed, wave, block, hub
 };
 int build_plan(const HostCSR& g, const Tuning& t, Plan& p);

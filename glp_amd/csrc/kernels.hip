@@ -855,9 +855,12 @@ __device__ __forceinline__ int32_t block_excl_sum(int32_t x, int32_t* total) {
 }
 constexpr int kHubThreads = 256;
 
-// dynamic LDS of k_hub_agg: the chunk's table plus the bucket histogram
-static inline size_t hub_agg_lds(int32_t chunk) {
-    return (size_t)next_pow2(2u * (uint32_t)chunk) * 2 * sizeof(uint32_t) + kMaxBuckets * sizeof(uint32_t);
+// dynamic LDS of k_hub_agg: the chunk's table plus the bucket histogram of
+// the launch's largest hub (buckets <= kMaxBuckets); the smaller the
+// histogram, the more workgroups share a CU
+static inline size_t hub_agg_lds(int32_t chunk, int32_t buckets = kMaxBuckets) {
+    return (size_t)next_pow2(2u * (uint32_t)chunk) * 2 * sizeof(uint32_t) +
+           (size_t)((buckets + 63) & ~63) * sizeof(uint32_t);
 }
 
 __device__ __forceinline__ uint32_t bucket_of(uint32_t hk, uint32_t P) {
@@ -871,7 +874,7 @@ __device__ __forceinline__ uint64_t bucket_lo(uint32_t p, uint32_t P) {
 template <class H, bool W, bool G>
 __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
     // dynamic LDS: table of next_pow2(2*chunk) keys and counts, then the
-    // bucket histogram (kMaxBuckets counters)
+    // bucket histogram (at least P counters for every hub of the launch)
     extern __shared__ __align__(16) uint32_t s_dyn[];
     __shared__ uint32_t s_npairs;
     const uint32_t cap_max = next_pow2(2u * (uint32_t)h.chunk);
@@ -1891,8 +1894,8 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
         const unsigned nc = (unsigned)p.hub_chunks.size();
         const unsigned np = (unsigned)p.hub_parts.size();
         const size_t lds_part = (size_t)next_pow2(2u * (uint32_t)h.part_keys) * 2 * sizeof(uint32_t);
-        hipLaunchKernelGGL((k_hub_agg<H, W, G>), dim3(nc), dim3(kHubThreads), hub_agg_lds(h.chunk), s, a,
-                           h);
+        hipLaunchKernelGGL((k_hub_agg<H, W, G>), dim3(nc), dim3(kHubThreads),
+                           hub_agg_lds(h.chunk, p.hub_max_buckets), s, a, h);
         mark(GLP_TIER_HUB, 1);
         mark(GLP_TIER_AUX, 0);
         hipLaunchKernelGGL(k_hub_scan, dim3(h.nhubs), dim3(kHubThreads), 0, s, a, h);

@@ -163,6 +163,7 @@ int build_plan(const HostCSR& g, const Tuning& t, Plan& p) {
             // one counting workgroup per bucket; empty buckets exit at once
             const int32_t nb = (int32_t)hub_buckets(d, t.part_keys);
             for (int32_t q = 0; q < nb; ++q) p.hub_parts.push_back(make_int2(h, q));
+            p.hub_max_buckets = std::max(p.hub_max_buckets, nb);
             p.hub_hist_base.push_back(p.hist_slots);
             p.hist_slots += (int64_t)nb * nch;
             p.edges[3] += d;
