@@ -1870,8 +1870,13 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
         } else if (cap <= 2048) {
             hipLaunchKernelGGL((k_block<H, W, G, 256, 4>), dim3(nb), dim3(256), lds, s, a,
                                g.d_block[c], cap);
-        } else {
+        } else if (cap <= 4096) {
             hipLaunchKernelGGL((k_block<H, W, G, 256>), dim3(nb), dim3(256), lds, s, a,
+                               g.d_block[c], cap);
+        } else {
+            // tables of 64 KB or more leave room for two workgroups per CU:
+            // 1024 threads of 4 gathers each keep the CU's waves full
+            hipLaunchKernelGGL((k_block<H, W, G, 1024, 4>), dim3(nb), dim3(1024), lds, s, a,
                                g.d_block[c], cap);
         }
     }
@@ -1942,6 +1947,8 @@ void init_kernels() {
     (void)hipFuncSetAttribute((const void*)k_block<HK, W, G, 512>,                              \
                               hipFuncAttributeMaxDynamicSharedMemorySize, lim);                 \
     (void)hipFuncSetAttribute((const void*)k_block<HK, W, G, 1024>,                             \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim);                 \
+    (void)hipFuncSetAttribute((const void*)k_block<HK, W, G, 1024, 4>,                          \
                               hipFuncAttributeMaxDynamicSharedMemorySize, lim);
 #define GLP_SETLDS_HUB(HK, W, G)                                                                \
     (void)hipFuncSetAttribute((const void*)k_hub_agg<HK, W, G>,                                 \
