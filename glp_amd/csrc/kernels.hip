@@ -1871,7 +1871,9 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
             hipLaunchKernelGGL((k_block<H, W, G, 256, 4>), dim3(nb), dim3(256), lds, s, a,
                                g.d_block[c], cap);
         } else if (cap <= 4096) {
-            hipLaunchKernelGGL((k_block<H, W, G, 256>), dim3(nb), dim3(256), lds, s, a,
+            // 32 KB tables: four workgroups per CU, of 512 threads of 4
+            // gathers each
+            hipLaunchKernelGGL((k_block<H, W, G, 512, 4>), dim3(nb), dim3(512), lds, s, a,
                                g.d_block[c], cap);
         } else {
             // tables of 64 KB or more leave room for two workgroups per CU:
@@ -1949,6 +1951,8 @@ void init_kernels() {
     (void)hipFuncSetAttribute((const void*)k_block<HK, W, G, 1024>,                             \
                               hipFuncAttributeMaxDynamicSharedMemorySize, lim);                 \
     (void)hipFuncSetAttribute((const void*)k_block<HK, W, G, 1024, 4>,                          \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim);                 \
+    (void)hipFuncSetAttribute((const void*)k_block<HK, W, G, 512, 4>,                           \
                               hipFuncAttributeMaxDynamicSharedMemorySize, lim);
 #define GLP_SETLDS_HUB(HK, W, G)                                                                \
     (void)hipFuncSetAttribute((const void*)k_hub_agg<HK, W, G>,                                 \
