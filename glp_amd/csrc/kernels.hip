@@ -854,6 +854,9 @@ __device__ __forceinline__ int32_t block_excl_sum(int32_t x, int32_t* total) {
     return base + incl - x;
 }
 constexpr int kHubThreads = 256;
+// k_hub_agg: a chunk's table takes 34 KB of LDS, so four workgroups share a
+// CU; 512 threads of 4 gathers each keep its waves full
+constexpr int kHubAggThreads = 512;
 
 // dynamic LDS of k_hub_agg: the chunk's table plus the bucket histogram of
 // the launch's largest hub (buckets <= kMaxBuckets); the smaller the
@@ -871,7 +874,8 @@ __device__ __forceinline__ uint64_t bucket_lo(uint32_t p, uint32_t P) {
     return (((uint64_t)p << 32) + P - 1) / P;
 }
 
-template <class H, bool W, bool G>
+// NT threads per workgroup, K gathers in flight per thread
+template <class H, bool W, bool G, int NT = kHubThreads, int K = 8>
 __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
     // dynamic LDS: table of next_pow2(2*chunk) keys and counts, then the
     // bucket histogram (at least P counters for every hub of the launch)
@@ -887,7 +891,7 @@ __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h
     const uint32_t P = hub_buckets(ve - vb, h.part_keys);
     uint32_t* hist = h.hs.hist + h.hist_base[hc.x];  // [bucket][chunk]
     if (!is_active<H>(a, v) || is_fixed<H>(a, v)) {  // block-uniform: the hub keeps its label
-        for (uint32_t q = threadIdx.x; q < P; q += kHubThreads) hist[(int64_t)q * nch + hc.y] = 0u;
+        for (uint32_t q = threadIdx.x; q < P; q += NT) hist[(int64_t)q * nch + hc.y] = 0u;
         if (threadIdx.x == 0) h.hs.chunk_cnt[blk] = 0u;
         return;
     }
@@ -895,16 +899,16 @@ __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h
     const int32_t e = min(b + h.chunk, ve);
     const uint32_t cap = min(max(next_pow2(2u * (uint32_t)(e - b)), 64u), cap_max);
     const uint32_t mask = cap - 1u;
-    for (uint32_t i = threadIdx.x; i < cap; i += kHubThreads) T[i] = 0ull;
-    for (uint32_t q = threadIdx.x; q < P; q += kHubThreads) s_hist[q] = 0u;
+    for (uint32_t i = threadIdx.x; i < cap; i += NT) T[i] = 0ull;
+    for (uint32_t q = threadIdx.x; q < P; q += NT) s_hist[q] = 0u;
     if (threadIdx.x == 0) s_npairs = 0u;
     __syncthreads();
-    count_edges<H, W, G, 8, kHubThreads>(a, v, b, e, threadIdx.x, T, mask);
+    count_edges<H, W, G, K, NT>(a, v, b, e, threadIdx.x, T, mask);
     __syncthreads();
     unsigned long long* out = h.hs.pairs + h.base[hc.x] + (int64_t)hc.y * h.chunk;
     // cap is a power of two >= 64, so each wave runs this loop whole; one
     // LDS atomic per wave reserves its pair slots
-    for (uint32_t i = threadIdx.x; i < cap; i += kHubThreads) {
+    for (uint32_t i = threadIdx.x; i < cap; i += NT) {
         const unsigned long long x = T[i];  // (label key << 32 | count): the pair itself
         const uint32_t k = (uint32_t)(x >> 32);
         const unsigned long long bal = __ballot(k != 0u);
@@ -919,7 +923,7 @@ __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h
         atomicAdd(&s_hist[bucket_of(fmix32(k), P)], 1u);
     }
     __syncthreads();
-    for (uint32_t q = threadIdx.x; q < P; q += kHubThreads) hist[(int64_t)q * nch + hc.y] = s_hist[q];
+    for (uint32_t q = threadIdx.x; q < P; q += NT) hist[(int64_t)q * nch + hc.y] = s_hist[q];
     if (threadIdx.x == 0) h.hs.chunk_cnt[blk] = s_npairs;
 }
 
@@ -1233,8 +1237,8 @@ __global__ __launch_bounds__(NT) void k_block(StepArgs a, const int32_t* __restr
     block_body<H, W, G, NT, K>(a, verts, cap_max, blockIdx.x);
 }
 template <class H, bool W, bool G>
-__global__ __launch_bounds__(kHubThreads) void k_hub_agg(StepArgs a, HubArgs h) {
-    hub_agg_body<H, W, G>(a, h, blockIdx.x);
+__global__ __launch_bounds__(kHubAggThreads) void k_hub_agg(StepArgs a, HubArgs h) {
+    hub_agg_body<H, W, G, kHubAggThreads, 4>(a, h, blockIdx.x);
 }
 __global__ __launch_bounds__(kHubThreads) void k_hub_scan(StepArgs a, HubArgs h) {
     hub_scan_body(a, h, blockIdx.x);
@@ -1901,7 +1905,7 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
         const unsigned nc = (unsigned)p.hub_chunks.size();
         const unsigned np = (unsigned)p.hub_parts.size();
         const size_t lds_part = (size_t)next_pow2(2u * (uint32_t)h.part_keys) * 2 * sizeof(uint32_t);
-        hipLaunchKernelGGL((k_hub_agg<H, W, G>), dim3(nc), dim3(kHubThreads),
+        hipLaunchKernelGGL((k_hub_agg<H, W, G>), dim3(nc), dim3(kHubAggThreads),
                            hub_agg_lds(h.chunk, p.hub_max_buckets), s, a, h);
         mark(GLP_TIER_HUB, 1);
         mark(GLP_TIER_AUX, 0);
