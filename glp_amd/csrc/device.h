@@ -28,7 +28,7 @@ struct Tuning {
     int32_t hub_chunk = 2048;      // edges per aggregation workgroup for hubs (deg > block_max_deg)
     int32_t part_keys = 2048;      // expected distinct labels per hub counting workgroup
     int32_t part_limit = 0;        // distinct labels before a hub range is halved (0: table/2)
-    int32_t block_threads = 256;   // workgroup size of the block tier
+    int32_t block_threads = 256;   // block-tier workgroup size (256: by table size, up to 1024)
 };
 int tuning_set(Tuning& t, const char* key, int32_t value);  // GLP_E_ARG on unknown key
 int tuning_get(const Tuning& t, const char* key, int32_t* value);

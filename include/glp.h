@@ -110,7 +110,9 @@ int glp_device_count(int32_t* count);
  * wavefront part of the wave tier off), half_lanes (16 or 32 lanes per
  * vertex there; 0 chooses per launch from the amount of work),
  * wave_max_deg, block_max_deg, hub_chunk, part_keys, part_limit,
- * block_threads.                                                             */
+ * block_threads (256, the default: a single graph's block tier picks 256,
+ * 512 or 1024 threads per workgroup by LDS table size, up to 16, 32 or
+ * 64 KB; 512 or 1024: that many for every table size).                     */
 int glp_tuning_set(const char* key, int32_t value);
 int glp_tuning_get(const char* key, int32_t* value);
 /* Autotuner (no reference counterpart; replaces hand-picked constants):
