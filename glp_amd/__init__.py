@@ -2,8 +2,9 @@
 
 This module is a thin ctypes mirror of the C ABI declared in include/glp.h
 and include/glp_tools.h.  It never computes labels itself: every LP step
-runs in the HIP kernels of libglp.so, and importing fails loudly when the
-library has not been built (python -m glp_amd.build).
+runs in the HIP kernels of libglp.so, and the first call into the library
+fails loudly (ImportError) when it has not been built (python -m
+glp_amd.build); importing the package alone does not need it.
 
 Reference interface mirrored (Gibyeng/GLP):
   * CSRGraph(path, startvalue, direct)       -> Graph.load_bin   (header/graph.h:69)
@@ -18,10 +19,36 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libglp.so")
 
-if not os.path.exists(LIB_PATH):
-    raise ImportError("libglp.so not built: run `python -m glp_amd.build` (looked in %s)" % _HERE)
+_SIGS = []  # (name, restype, argtypes) applied when the library loads
 
-_lib = ctypes.CDLL(LIB_PATH)
+
+class _Library:
+    """libglp.so, loaded on first use.  Importing the package never needs the
+    library (so the build module runs from a clean tree); the first call into
+    it raises ImportError when it has not been built."""
+
+    handle = None
+
+    def __getattr__(self, name):
+        if _Library.handle is None:
+            _load()
+        return getattr(_Library.handle, name)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libglp.so not built: run `python -m glp_amd.build` (looked in %s)" % _HERE)
+    h = ctypes.CDLL(LIB_PATH)
+    for name, restype, argtypes in _SIGS:
+        f = getattr(h, name, None)
+        if f is None:
+            continue
+        f.restype = restype
+        f.argtypes = list(argtypes)
+    _Library.handle = h
+
+
+_lib = _Library()
 
 CLASSIC, LAYERED, SLPA, USER, COUNTMIN, FREQUENT, BUFFER, SEEDED = 1, 2, 3, 4, 5, 6, 7, 8
 METHOD_NAMES = {CLASSIC: "classic", LAYERED: "layered", SLPA: "slpa", USER: "user",
@@ -73,12 +100,7 @@ class JobStats(ctypes.Structure):
 
 
 def _sig(name, restype, *argtypes):
-    f = getattr(_lib, name, None)
-    if f is None:
-        return None
-    f.restype = restype
-    f.argtypes = list(argtypes)
-    return f
+    _SIGS.append((name, restype, argtypes))
 
 
 _vp = ctypes.c_void_p
@@ -150,7 +172,9 @@ TUNING_KEYS = ("pack_max_deg", "half_max_deg", "half_lanes", "wave_max_deg", "bl
 
 def lib():
     """The loaded libglp.so handle."""
-    return _lib
+    if _Library.handle is None:
+        _load()
+    return _Library.handle
 
 
 _sig("glp_autotune", ctypes.c_int, _vp, ctypes.POINTER(Params), ctypes.c_int32, ctypes.c_char_p,

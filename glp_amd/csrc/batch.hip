@@ -113,6 +113,7 @@ extern "C" int glp_batch_run(glp_batch* b, int32_t steps, int32_t mode, double* 
             if ((rc = session_sync(s)) != GLP_OK) return rc;
         }
     } else {
+        if (!b->sess) return set_error(GLP_E_STATE, "the batch has no union session (autotune failed)");
         rc = mode == 1 ? session_step_graph(b->sess, steps) : session_step(b->sess, steps);
         if (rc != GLP_OK) return rc;
         if ((rc = session_sync(b->sess)) != GLP_OK) return rc;
@@ -130,6 +131,7 @@ extern "C" int glp_batch_labels(glp_batch* b, int32_t graph, int32_t* out) {
     const int32_t n = b->graphs[graph]->csr.n;
     if (n && !out) return set_error(GLP_E_ARG, "null output");
     if (b->last_mode == 2 && !b->plain.empty()) return session_labels(b->plain[graph], out);
+    if (!b->sess) return set_error(GLP_E_STATE, "the batch has no union session (autotune failed)");
     int rc = session_sync(b->sess);
     if (rc != GLP_OK) return rc;
     if (n == 0) return GLP_OK;
@@ -142,6 +144,7 @@ extern "C" int glp_batch_labels(glp_batch* b, int32_t graph, int32_t* out) {
 extern "C" int glp_batch_reset(glp_batch* b) {
     clear_error();
     if (!b) return set_error(GLP_E_ARG, "null batch");
+    if (!b->sess) return set_error(GLP_E_STATE, "the batch has no union session (autotune failed)");
     int rc = session_reset(b->sess);
     if (rc != GLP_OK) return rc;
     for (auto* s : b->plain)
@@ -152,12 +155,16 @@ extern "C" int glp_batch_reset(glp_batch* b) {
 extern "C" int glp_batch_autotune(glp_batch* b, int32_t steps, double* best_ms) {
     clear_error();
     if (!b || steps < 1) return set_error(GLP_E_ARG, "bad batch autotune arguments");
-    // the union session's scratch was sized by the old plan: rebuild it
-    delete b->sess;
-    b->sess = nullptr;
+    // on failure the union keeps its plan, which the current session fits
     int rc = glp_autotune(b->uni.get(), &b->p, steps, nullptr, best_ms);
     if (rc != GLP_OK) return rc;
-    return session_create(b->uni.get(), &b->p, &b->sess);
+    // the union session's scratch was sized by the old plan: replace it (a
+    // failed creation leaves no session, which run/labels/reset report)
+    glp_session* s = nullptr;
+    rc = session_create(b->uni.get(), &b->p, &s);
+    delete b->sess;
+    b->sess = s;
+    return rc;
 }
 
 extern "C" void glp_batch_free(glp_batch* b) { delete b; }

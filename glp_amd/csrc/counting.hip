@@ -158,6 +158,15 @@ __global__ __launch_bounds__(64) void k_table(StepArgs a, int32_t cap) {
         atomicAdd(&a.changes[blockIdx.x & (kChangeSlots - 1)], (unsigned long long)__popcll(bal));
 }
 
+// k_countmin takes rows * width * 4 bytes of dynamic LDS, up to 5 * 4096 * 4
+// = 80 KB (the limits check_params enforces), above the 64 KB a launch gets
+// without the attribute; k_table takes at most 64 * 2 * 64 * 4 = 32 KB.
+void init_counting_kernels() {
+    const int lim = 5 * 4096 * 4;
+    (void)hipFuncSetAttribute((const void*)k_countmin<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void*)k_countmin<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+}
+
 int enqueue_counting_step(glp_session* s) {
     const DeviceGraph& dg = *s->g->dev;
     StepArgs a{};

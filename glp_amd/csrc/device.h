@@ -165,6 +165,9 @@ enum class Method { Classic, Layered, Slpa, User, Seeded };
 
 // One-time kernel attribute set-up (LDS limits); call before any capture.
 void init_kernels();
+// The same for the count-min and table kernels (counting.hip); init_kernels
+// calls it.
+void init_counting_kernels();
 
 // Enqueues the counting part of one step (all tiers) on stream s.
 // ev (may be null) holds 2*GLP_NTIERS events for per-tier timing.

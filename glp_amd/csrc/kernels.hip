@@ -1942,6 +1942,7 @@ void init_kernels() {
     static bool done = false;
     if (done) return;
     done = true;
+    init_counting_kernels();
     const int lim = 160 * 1024 - 1024;
 #define GLP_SETLDS(HK, W, G)                                                                    \
     (void)hipFuncSetAttribute((const void*)k_block<HK, W, G, 256>,                              \

@@ -214,8 +214,8 @@ int glp_session_slpa_memory(glp_session* s, int32_t* out);
  * of such labels, and the dominant label (highest count, then highest id).  */
 int glp_session_slpa_post(glp_session* s, int32_t kmax, int32_t* dominant,
                           int32_t* nsets, int32_t* sets);
-/* Ids of the session's hot kernels and their HIP-event durations (ms) over
- * the last profiled steps, for roofline accounting.                          */
+/* Waits for the session's queued work and frees its device state (the
+ * reference's free_gmem, prop/classicLprop.cuh:107-116).                    */
 int glp_session_free(glp_session* s);
 
 /* ---- batched: many graphs in one persistent launch --------------------- */

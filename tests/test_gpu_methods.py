@@ -1,5 +1,6 @@
 """GPU parity of the non-classic methods (layered, SLPA, count-min, frequent,
-buffer, user hooks) and of the batched and served forms against solo runs."""
+buffer, user hooks) and of the batched and served forms, against the oracle
+and against solo runs."""
 import numpy as np
 import pytest
 
@@ -79,6 +80,15 @@ def test_slpa_memory_and_post(rmat14, T):
     assert np.array_equal(labels, dw)
 
 
+def test_slpa_hand_trace():
+    """The hand-traced three-step SLPA case of the oracle's tests (seed 3)."""
+    g = G.Graph.from_edges(5, [(0, 1), (1, 2), (2, 3), (3, 4), (1, 3)])
+    with G.Session(g, method=G.SLPA, seed=3, slpa_T=3) as s:
+        s.step(3)
+        mem = s.slpa_memory(4)
+    assert mem.tolist() == [[0, 1, 2, 3, 4], [1, 3, 3, 4, 3], [1, 3, 3, 3, 3], [3, 4, 4, 3, 4]]
+
+
 def test_slpa_lfr_graph():
     g = G.Graph.lfr(1 << 13, seed=5)
     off, nbr, _ = g.csr()
@@ -100,6 +110,36 @@ def test_countmin_matches_oracle(rmat14, rows, width, aliased):
     for t in range(3):
         prev = O.countmin_step(off, nbr, prev, rows, width, aliased)
         assert np.array_equal(dump[t], prev), "step %d" % (t + 1)
+
+
+@pytest.mark.parametrize("a,b,ca,rows,aliased,want", [
+    # the oracle's hand-derived width-4 collision cases (test_oracle.py)
+    (1, 4, 2, 3, 0, 1), (1, 4, 2, 1, 0, 4),
+    (2, 17, 2, 3, 0, 2), (2, 17, 2, 3, 1, 17), (2, 17, 2, 4, 1, 17),
+    (1, 15, 1, 3, 0, 15), (1, 15, 1, 3, 1, 1), (1, 15, 1, 4, 1, 15)])
+def test_countmin_hand_collisions(a, b, ca, rows, aliased, want):
+    """A star centre 0 sees leaf a on ca parallel edges and leaf b on one; at
+    step 1 (labels = ids) its count-min winner at width 4 is argued by hand
+    from the row hashes' cells."""
+    g = G.Graph.from_edges(b + 1, [(0, a)] * ca + [(0, b)])
+    labels, _ = G.run(g, 1, method=G.COUNTMIN, cm_rows=rows, cm_width=4, cm_aliased=aliased)
+    assert labels[0] == want
+    off, nbr, _ = g.csr()
+    assert np.array_equal(labels, O.countmin_step(off, nbr, np.arange(b + 1, dtype=np.int32), rows, 4,
+                                                  aliased))
+
+
+def test_countmin_largest_sketch():
+    """rows x width = 5 x 4096 (80 KB of LDS, the largest accepted shape)."""
+    g = G.Graph.rmat(11, 16, seed=34)
+    off, nbr, _ = g.csr()
+    labels, _ = G.run(g, 2, method=G.COUNTMIN, cm_rows=5, cm_width=4096)
+    prev = np.arange(len(off) - 1, dtype=np.int32)
+    for _ in range(2):
+        prev = O.countmin_step(off, nbr, prev, 5, 4096, 0)
+    assert np.array_equal(labels, prev)
+    with pytest.raises(G.GLPError):
+        G.run(g, 1, method=G.COUNTMIN, cm_rows=5, cm_width=4097)
 
 
 def test_countmin_weighted(rmat13w):
@@ -210,19 +250,48 @@ def mix_graphs(count, seed, nmax=6000):
     return gs
 
 
+def oracle_labels(g, steps, method, kw, weighted=False):
+    off, nbr, w = g.csr()
+    w = w if weighted else None
+    if method == G.CLASSIC:
+        return O.classic(off, nbr, steps, w=w)[-1]
+    if method == G.LAYERED:
+        return O.layered(off, nbr, steps, kw["gamma"][0], kw["gamma"][1], w=w)[0][-1]
+    prev = np.arange(len(off) - 1, dtype=np.int32)
+    for _ in range(steps):
+        prev = O.table_step(off, nbr, prev, kw["table_cap"], method == G.BUFFER, w=w)
+    return prev
+
+
+# small thresholds put the RMAT graph's high-degree vertices in the hub tier
+# (degree > 32) and the planted graphs' in the block tier
+HUB_TUNING = dict(pack_max_deg=8, wave_max_deg=16, block_max_deg=32, hub_chunk=256, part_keys=256)
+
+
 @pytest.mark.parametrize("method,kw", [(G.CLASSIC, {}), (G.LAYERED, {"gamma": (1, 4)}),
                                        (G.FREQUENT, {"table_cap": 8})])
-def test_batch_equals_alone(method, kw):
-    gs = mix_graphs(24, 5)
-    gs.append(G.Graph.rmat(12, 16, seed=2))     # one graph with hubs
-    b = G.Batch(gs, method=method, **kw)
-    for mode in (0, 1, 2):
-        b.reset()
-        b.run(6, mode=mode)
-        for i, g in enumerate(gs):
+@pytest.mark.parametrize("tuned", [False, True])
+def test_batch_equals_alone_and_oracle(method, kw, tuned):
+    """Every graph of a batch (union stepped in one launch per tier, its HIP
+    graph replay, and the plain per-graph form) matches the oracle and the
+    graph run alone."""
+    with G.tuning(**(HUB_TUNING if tuned else {})):
+        gs = mix_graphs(24, 5)
+        gs.append(G.Graph.rmat(12, 16, seed=2))     # max degree 1339: hub tier when tuned
+        if tuned:
+            _, t = G.run(gs[-1], 1, method=method, **kw)
+            assert method == G.FREQUENT or t["tier_vertices"]["hub"] > 0
+        want = [oracle_labels(g, 6, method, kw) for g in gs]
+        b = G.Batch(gs, method=method, **kw)
+        for mode in (0, 1, 2):
+            b.reset()
+            b.run(6, mode=mode)
+            for i, g in enumerate(gs):
+                assert np.array_equal(b.labels(i), want[i]), "mode %d graph %d" % (mode, i)
+        for i, g in enumerate(gs[:4] + gs[-1:]):
             alone, _ = G.run(g, 6, method=method, **kw)
-            assert np.array_equal(b.labels(i), alone), "mode %d graph %d" % (mode, i)
-    b.free()
+            assert np.array_equal(alone, want[gs.index(g)])
+        b.free()
 
 
 def test_batch_rejects_stream_methods():
@@ -233,26 +302,36 @@ def test_batch_rejects_stream_methods():
 
 # ------------------------------------------------------------------- served ----
 
-def test_server_results_equal_alone_in_the_mix():
+@pytest.mark.parametrize("tuned", [False, True])
+def test_server_results_equal_alone_in_the_mix(tuned):
+    """Weighted classic and layered (gamma 1/4) jobs served together under a
+    small budget: each result matches the oracle and the job run alone.
+    Tuned: the folded hub launches run (every graph has degrees > 32)."""
     rng = np.random.default_rng(3)
     graphs, params, steps = [], [], []
-    for i in range(12):
-        scale = int(rng.integers(8, 13))
-        g = G.Graph.rmat(scale, int(rng.integers(8, 17)), seed=100 + i, wmax=8)
-        layered = i % 2 == 1
-        p = G.make_params(method=G.LAYERED if layered else G.CLASSIC, gamma=(1, 4) if layered else (0, 1),
-                          weighted=True)
-        graphs.append(g)
-        params.append(p)
-        steps.append(int(rng.integers(3, 9)))
-    sv = G.Server(budget_bytes=64 << 20, max_slots=4)   # small budget: jobs queue for memory
-    ids = [sv.submit(g, s, params=p) for g, s, p in zip(graphs, steps, params)]
-    for jid, g, s, p in zip(ids, graphs, steps, params):
-        got, st = sv.wait(jid, g.n)
-        alone, _ = G.run(g, s, params=p)
-        assert st.state == 2 and st.steps == s
-        assert np.array_equal(got, alone)
-    sv.close()
+    with G.tuning(**(HUB_TUNING if tuned else {})):
+        for i in range(12):
+            scale = int(rng.integers(8, 13))
+            g = G.Graph.rmat(scale, int(rng.integers(8, 17)), seed=100 + i, wmax=8)
+            layered = i % 2 == 1
+            p = G.make_params(method=G.LAYERED if layered else G.CLASSIC,
+                              gamma=(1, 4) if layered else (0, 1), weighted=True)
+            graphs.append(g)
+            params.append(p)
+            steps.append(int(rng.integers(3, 9)))
+        if tuned:
+            assert all(g.info()["max_degree"] > HUB_TUNING["block_max_deg"] for g in graphs)
+        sv = G.Server(budget_bytes=64 << 20, max_slots=4)   # small budget: jobs queue for memory
+        ids = [sv.submit(g, s, params=p) for g, s, p in zip(graphs, steps, params)]
+        for jid, g, s, p in zip(ids, graphs, steps, params):
+            got, st = sv.wait(jid, g.n)
+            assert st.state == 2 and st.steps == s
+            kw = {"gamma": (p.gamma_num, p.gamma_den)}
+            want = oracle_labels(g, s, G.LAYERED if p.method == G.LAYERED else G.CLASSIC, kw, weighted=True)
+            assert np.array_equal(got, want), "job %d" % jid
+            alone, _ = G.run(g, s, params=p)
+            assert np.array_equal(got, alone)
+        sv.close()
 
 
 def test_server_budget_and_cancel():

@@ -160,6 +160,40 @@ def test_slpa_memory_rows_are_neighbour_labels():
                 assert mem[t, v] in set(mem[:t, nb].ravel().tolist())
 
 
+def _splitmix_first(state):
+    """First output of Vigna's SplitMix64 from `state` (independent of the
+    oracle; pinned by the published answers below)."""
+    m = (1 << 64) - 1
+    z = (state + 0x9e3779b97f4a7c15) & m
+    z = ((z ^ (z >> 30)) * 0xbf58476d1ce4e5b9) & m
+    z = ((z ^ (z >> 27)) * 0x94d049bb133111eb) & m
+    return z ^ (z >> 31)
+
+
+def test_slpa_hand_trace_three_steps():
+    """Graph 0-1, 1-2, 2-3, 3-4, 1-3; seed 3.  Speaker u at step t shows
+    M_u[R mod t], R = low 32 bits of SplitMix64's first output from state
+    (seed<<40) ^ (t<<32) ^ u; the listener takes the most frequent shown
+    label, ties to the highest.  The memory rows below are traced by hand."""
+    assert _splitmix_first(0) == 0xe220a8397b1dcdaf
+    assert _splitmix_first(1234567) == 6457827717110365317
+    seed = 3
+    rows = {t: [(_splitmix_first((seed << 40) ^ (t << 32) ^ u) & 0xFFFFFFFF) % t for u in range(5)]
+            for t in (2, 3)}
+    assert rows == {2: [1, 0, 1, 0, 1], 3: [2, 1, 2, 1, 0]}
+    for t in (1, 2, 3):
+        assert [O.stream_u32(seed, t, u) % t for u in range(5)] == ([0] * 5 if t == 1 else rows[t])
+    off, nbr, _ = O.csr_from_edges(5, [(0, 1), (1, 2), (2, 3), (3, 4), (1, 3)])
+    mem = O.slpa(off, nbr, seed=seed, T=3)
+    # t=1: every speaker shows M[0] = id; v1 sees {0,2,3} -> 3, v3 {2,4,1} -> 4
+    # t=2: shown = [M1[0], M0[1], M1[2], M0[3], M1[4]] = [1,1,3,3,3]
+    # t=3: shown = [M2[0], M1[1], M2[2], M1[3], M0[4]] = [1,3,3,4,4]
+    assert mem.tolist() == [[0, 1, 2, 3, 4],
+                            [1, 3, 3, 4, 3],
+                            [1, 3, 3, 3, 3],
+                            [3, 4, 4, 3, 4]]
+
+
 def test_slpa_post_hand():
     mem = np.array([[3], [3], [5], [5], [5], [7], [3]], dtype=np.int32)
     dom, ns, sets = O.slpa_post(mem, 1, 10, 4)
@@ -213,6 +247,79 @@ def test_countmin_without_collisions_is_classic():
     for aliased in (0, 1):
         assert np.array_equal(O.countmin_step(off, nbr, prev, 3, W, aliased),
                               O.classic_step(off, nbr, prev))
+
+
+# Hand-derived collision cases at width 4.  Vertex 0 is a star centre whose
+# leaves show label a (count ca) and label b > a (count cb); every other
+# vertex is a leaf.  The cells each key touches come from _cm_hash_py, and
+# the expected winner is argued from them by hand below (C_r = row r's cell
+# sums; est = minimum over the cells a key touches; ties go to the higher
+# label).  rows are method/countmin.cuh:145-263: proper rows use h_r in row
+# r; aliased d=3 adds h_3's increment to row 2 (:185-191), aliased d=4 adds
+# h_3's cell of row 2 twice (:210-219).
+CM_W = 4
+
+
+def _cm_cells(label):
+    return [_cm_hash_py(r, label + 1) % CM_W for r in range(5)]
+
+
+def _cm_centre(counts, rows, aliased):
+    """The oracle's label for the centre of a star whose leaves show
+    `counts` = {label: multiplicity}."""
+    prev = [0]
+    for lab, c in counts.items():
+        prev += [lab] * c
+    n = len(prev)
+    off, nbr, _ = O.csr_from_edges(n, [(0, i) for i in range(1, n)])
+    return int(O.countmin_step(off, nbr, np.array(prev, dtype=np.int32), rows, CM_W, aliased)[0])
+
+
+def test_countmin_rows_beyond_the_first_decide():
+    """a=1 (x2), b=4 (x1): both keys share their row-0 cell (C_0 = 3 for
+    both), rows 1 and 2 separate them.  Proper min over rows: est(a) = 2,
+    est(b) = 1, so a wins; row 0 alone would tie 3:3 and give b."""
+    a, b = 1, 4
+    A, B = _cm_cells(a), _cm_cells(b)
+    assert A[0] == B[0] and A[1] != B[1] and A[2] != B[2] and A[4] != B[4]
+    for rows in (2, 3, 5):
+        assert _cm_centre({a: 2, b: 1}, rows, 0) == a
+    assert _cm_centre({a: 2, b: 1}, 1, 0) == b   # row 0 only: the 3:3 tie
+    # d=4 proper: rows 3 collide (A[3] == B[3]) but rows 1, 2 still separate
+    assert A[3] == B[3] and _cm_centre({a: 2, b: 1}, 4, 0) == a
+
+
+def test_countmin_aliased_row_two_crosses_keys():
+    """a=2 (x2), b=17 (x1), cells a=[1,2,0,..], b=[1,0,2,..].  Proper d=3:
+    est(a) = min(3,2,2) = 2 > est(b) = min(3,1,1) = 1, a wins.  Aliased d=3
+    puts every key's h_1 and h_2 increments in row 1, where a's h_1 cell (2)
+    is b's h_2 cell and a's h_2 cell (0) is b's h_1 cell: both cells hold 3,
+    so est(a) = est(b) = 3 and the tie goes to b.  Aliased d=4 adds h_2
+    twice: C_1[0] = 4+1 = 5, C_1[2] = 2+2 = 4, still est 3:3, b."""
+    a, b = 2, 17
+    A, B = _cm_cells(a), _cm_cells(b)
+    assert A[0] == B[0] and B[1] == A[2] and B[2] == A[1]
+    assert A[1] != B[1] and A[2] != B[2] and A[3] != B[3]
+    assert _cm_centre({a: 2, b: 1}, 3, 0) == a
+    assert _cm_centre({a: 2, b: 1}, 4, 0) == a
+    assert _cm_centre({a: 2, b: 1}, 3, 1) == b
+    assert _cm_centre({a: 2, b: 1}, 4, 1) == b
+
+
+def test_countmin_aliased_d3_and_d4_differ():
+    """a=1, b=15, one edge each, cells a=[3,0,0,..], b=[3,0,2,..].  Proper:
+    rows 0 and 1 collide (2:2), row 2 separates (1:1): tie, b.  Aliased d=3:
+    row 1 gets a's h_1 and h_2 (both cell 0) and b's h_1 (cell 0) and h_2
+    (cell 2): C_1[0] = 3, C_1[2] = 1, so est(a) = min(2,3,3) = 2 and
+    est(b) = min(2,3,1) = 1: a wins although b is higher.  Aliased d=4 adds
+    h_2 twice: C_1[0] = 1+2+1 = 4, C_1[2] = 2, est(a) = 2 = est(b): b."""
+    a, b = 1, 15
+    A, B = _cm_cells(a), _cm_cells(b)
+    assert A[0] == B[0] and A[1] == A[2] == B[1] and B[2] != A[1] and A[3] != B[3]
+    for rows in (3, 4, 5):
+        assert _cm_centre({a: 1, b: 1}, rows, 0) == b
+    assert _cm_centre({a: 1, b: 1}, 3, 1) == a
+    assert _cm_centre({a: 1, b: 1}, 4, 1) == b
 
 
 def test_countmin_width_one_is_highest_label():
