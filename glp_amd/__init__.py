@@ -96,7 +96,8 @@ class Timing(ctypes.Structure):
 class JobStats(ctypes.Structure):
     _fields_ = [("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double),
                 ("done_ms", ctypes.c_double), ("bytes", ctypes.c_int64),
-                ("steps", ctypes.c_int32), ("state", ctypes.c_int32)]
+                ("steps", ctypes.c_int32), ("state", ctypes.c_int32),
+                ("start_ms", ctypes.c_double)]
 
 
 def _sig(name, restype, *argtypes):
@@ -160,6 +161,15 @@ _sig("glp_server_submit", ctypes.c_int, _vp, _vp, ctypes.POINTER(Params), ctypes
 _sig("glp_server_poll", ctypes.c_int, _vp, ctypes.c_int64, ctypes.POINTER(JobStats))
 _sig("glp_server_wait", ctypes.c_int, _vp, ctypes.c_int64, _c_i32p, ctypes.POINTER(JobStats))
 _sig("glp_server_cancel", ctypes.c_int, _vp, ctypes.c_int64)
+_sig("glp_server_release", ctypes.c_int, _vp, ctypes.c_int64)
+_sig("glp_server_add_graph", ctypes.c_int, _vp, _vp, _c_i32p)
+_sig("glp_server_add_rmat", ctypes.c_int, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+     ctypes.c_double, ctypes.c_uint64, ctypes.c_int32, _c_i32p)
+_sig("glp_server_graph_info", ctypes.c_int, _vp, ctypes.c_int32, _c_i32p, _c_i64p, _c_i32p, _c_i64p)
+_sig("glp_server_graph_csr", ctypes.c_int, _vp, ctypes.c_int32, _c_i32p, _c_i32p, _c_u32p)
+_sig("glp_server_remove_graph", ctypes.c_int, _vp, ctypes.c_int32)
+_sig("glp_server_submit_resident", ctypes.c_int, _vp, ctypes.c_int32, ctypes.POINTER(Params), ctypes.c_int32,
+     _c_i64p)
 _sig("glp_server_free", ctypes.c_int, _vp)
 
 
@@ -619,3 +629,43 @@ class Server:
 
     def cancel(self, jid):
         _check(_lib.glp_server_cancel(self._h, jid))
+
+    def release(self, jid):
+        """Forgets an ended job's labels and stats."""
+        _check(_lib.glp_server_release(self._h, jid))
+
+    def add_graph(self, graph):
+        """Uploads `graph` once for any number of jobs; returns its id."""
+        gid = ctypes.c_int32()
+        _check(_lib.glp_server_add_graph(self._h, graph.handle, ctypes.byref(gid)))
+        return gid.value
+
+    def add_rmat(self, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, wmax=0):
+        """Builds Graph.rmat(scale, ..., degree_sort=False, wmax)'s graph in
+        HBM; returns its id."""
+        gid = ctypes.c_int32()
+        _check(_lib.glp_server_add_rmat(self._h, scale, edge_factor, a, b, c, seed, wmax, ctypes.byref(gid)))
+        return gid.value
+
+    def graph_info(self, gid):
+        n, m, d, b = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int64()
+        _check(_lib.glp_server_graph_info(self._h, gid, ctypes.byref(n), ctypes.byref(m), ctypes.byref(d),
+                                          ctypes.byref(b)))
+        return {"n": n.value, "m": m.value, "max_degree": d.value, "bytes": b.value}
+
+    def graph_csr(self, gid, weights=False):
+        inf = self.graph_info(gid)
+        off = np.zeros(inf["n"] + 1, dtype=np.int32)
+        nbr = np.zeros(max(inf["m"], 1), dtype=np.int32)
+        w = np.zeros(max(inf["m"], 1), dtype=np.uint32) if weights else None
+        _check(_lib.glp_server_graph_csr(self._h, gid, _i32(off), _i32(nbr), _u32(w) if weights else None))
+        return off, nbr[:inf["m"]], (w[:inf["m"]] if weights else None)
+
+    def remove_graph(self, gid):
+        _check(_lib.glp_server_remove_graph(self._h, gid))
+
+    def submit_resident(self, gid, steps, params=None, **kw):
+        p = params if params is not None else make_params(**kw)
+        jid = ctypes.c_int64()
+        _check(_lib.glp_server_submit_resident(self._h, gid, ctypes.byref(p), steps, ctypes.byref(jid)))
+        return jid.value

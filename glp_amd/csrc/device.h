@@ -207,6 +207,38 @@ inline int launch_zero_bytes(void* p, int64_t count, hipStream_t s) {
     return launch_fill_i32(static_cast<int32_t*>(p), 0, count / 4, s);
 }
 
+// One layered job's written labels and write-side volumes (folded volume
+// launches of the serving path).
+struct VolumeJob {
+    const int32_t* L;
+    int32_t* V;
+    int32_t n;
+    int32_t pad;
+};
+// Workgroups a job of n vertices needs in the folded zero and volume launches.
+void volume_job_blocks(int32_t n, int32_t* zero_blocks, int32_t* volume_blocks);
+// zero: V = 0 for every job (before the tiers); else V += histogram of L
+// (after them).  cum: nj + 1 prefix sums of the job's workgroups; grid = cum[nj].
+int launch_multi_volume(bool zero, const VolumeJob* jobs, const int32_t* cum, int32_t nj, int32_t grid,
+                        hipStream_t s);
+
+// Device CSR built on the device (gen_device.hip); n + 1 offsets, m
+// neighbours and m weights (null when unweighted), all hipMalloc'ed.
+struct DeviceCSR {
+    int32_t n = 0;
+    int64_t m = 0;
+    int32_t* off = nullptr;
+    int32_t* nbr = nullptr;
+    uint32_t* w = nullptr;
+    int32_t max_degree = 0;
+    uint32_t wmax = 0;
+};
+// The graph glp_gen_rmat(scale, edge_factor, a, b, c, seed, degree_sort = 0,
+// wmax) builds on the host (same draws, same CSR), built in HBM on stream s
+// (synchronises s).
+int gen_rmat_device(int32_t scale, int32_t edge_factor, double a, double b, double c, uint64_t seed,
+                    int32_t wmax, hipStream_t s, DeviceCSR* out);
+
 // Folded step for nj jobs with layered/classic hooks (gamma 0 = classic).
 // jobs and cum are device arrays; grid (host) holds each launch's size.
 int launch_multi(bool weighted, const JobDesc* jobs, const int32_t* cum, int32_t nj,

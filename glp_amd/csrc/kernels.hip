@@ -1672,13 +1672,13 @@ constexpr int kVolThreads = 256;
 constexpr int kVolChunk = 16384;
 constexpr uint32_t kVolSlots = 2048;
 constexpr int kVolProbe = 8;
-__global__ __launch_bounds__(kVolThreads) void k_volume(const int32_t* __restrict__ L,
-                                                        int32_t* __restrict__ V, int32_t n) {
+__device__ __forceinline__ void volume_body(const int32_t* __restrict__ L, int32_t* __restrict__ V,
+                                            int32_t n, uint32_t blk) {
     __shared__ unsigned long long T[kVolSlots];
     const int lane = lane_id();
     for (uint32_t i = threadIdx.x; i < kVolSlots; i += kVolThreads) T[i] = 0ull;
     __syncthreads();
-    const int64_t c0 = (int64_t)blockIdx.x * kVolChunk;
+    const int64_t c0 = (int64_t)blk * kVolChunk;
     const int64_t c1 = min((int64_t)n, c0 + kVolChunk);
     for (int64_t r = c0 + (threadIdx.x & ~63); r < c1; r += kVolThreads) {  // wave-uniform rows
         const int64_t v = r + lane;
@@ -1715,6 +1715,42 @@ __global__ __launch_bounds__(kVolThreads) void k_volume(const int32_t* __restric
         const unsigned long long x = T[i];
         if (x) atomicAdd(&V[(uint32_t)(x >> 32) - 1u], (int32_t)(uint32_t)x);
     }
+}
+__global__ __launch_bounds__(kVolThreads) void k_volume(const int32_t* __restrict__ L,
+                                                        int32_t* __restrict__ V, int32_t n) {
+    volume_body(L, V, n, blockIdx.x);
+}
+
+// Folded forms for the serving path: one launch zeroes (before the tiers)
+// or fills (after them) the write-side volumes of every layered job of a
+// step; cum[j] is job j's first workgroup, as for k_multi.
+constexpr int kZeroPerBlock = 256 * 16;
+__global__ __launch_bounds__(256) void k_multi_zero(const VolumeJob* __restrict__ jobs,
+                                                    const int32_t* __restrict__ cum, int32_t nj) {
+    const int32_t j = job_of(cum, nj, blockIdx.x);
+    const VolumeJob& jb = jobs[j];
+    const int64_t b0 = (int64_t)(blockIdx.x - (uint32_t)cum[j]) * kZeroPerBlock;
+    const int64_t b1 = min((int64_t)jb.n, b0 + kZeroPerBlock);
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += 256) jb.V[i] = 0;
+}
+__global__ __launch_bounds__(kVolThreads) void k_multi_volume(const VolumeJob* __restrict__ jobs,
+                                                              const int32_t* __restrict__ cum, int32_t nj) {
+    const int32_t j = job_of(cum, nj, blockIdx.x);
+    volume_body(jobs[j].L, jobs[j].V, jobs[j].n, blockIdx.x - (uint32_t)cum[j]);
+}
+
+void volume_job_blocks(int32_t n, int32_t* zero_blocks, int32_t* volume_blocks) {
+    *zero_blocks = (int32_t)(((int64_t)n + kZeroPerBlock - 1) / kZeroPerBlock);
+    *volume_blocks = (int32_t)(((int64_t)n + kVolChunk - 1) / kVolChunk);
+}
+
+int launch_multi_volume(bool zero, const VolumeJob* jobs, const int32_t* cum, int32_t nj, int32_t grid,
+                        hipStream_t s) {
+    if (nj == 0 || grid == 0) return GLP_OK;
+    if (zero) hipLaunchKernelGGL(k_multi_zero, dim3(grid), dim3(256), 0, s, jobs, cum, nj);
+    else hipLaunchKernelGGL(k_multi_volume, dim3(grid), dim3(kVolThreads), 0, s, jobs, cum, nj);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "folded volume launch: %s", hipGetErrorString(e));
 }
 
 // Seeded LP's L_0 (header/kernel.cuh:453-466): v for a seed, -1 otherwise.

@@ -51,6 +51,8 @@ struct glp_session {
 namespace glp {
 int ensure_device();
 int check_params(const HostCSR& g, const glp_params* p);
+// The same from the graph's shape: n, largest degree, largest weight.
+int check_params_shape(int32_t n, int32_t max_degree, uint32_t wmax, bool weighted, const glp_params* p);
 size_t session_bytes(const HostCSR& g, const Plan& p, const glp_params& prm);
 int session_create(glp_graph* g, const glp_params* p, glp_session** out);
 int session_reset(glp_session* s);

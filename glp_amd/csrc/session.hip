@@ -139,6 +139,13 @@ static bool counting_method(int32_t m) {
 }
 
 int check_params(const HostCSR& g, const glp_params* p) {
+    uint32_t wmax = 1;
+    if (p && p->weighted)
+        for (uint32_t w : g.weights) wmax = std::max(wmax, w);
+    return check_params_shape(g.n, g.max_degree, wmax, g.weighted(), p);
+}
+
+int check_params_shape(int32_t n, int32_t max_degree, uint32_t wmax_in, bool weighted, const glp_params* p) {
     if (!p) return set_error(GLP_E_ARG, "null params");
     for (int i = 0; i < 5; ++i)
         if (p->reserved[i]) return set_error(GLP_E_ARG, "reserved params fields must be zero");
@@ -150,12 +157,10 @@ int check_params(const HostCSR& g, const glp_params* p) {
         return set_error(GLP_E_UNSUPPORTED,
                          "the frontier is exact for classic LP only (layered volumes and SLPA speakers "
                          "change every label's score each step)");
-    if (p->weighted && !g.weighted()) return set_error(GLP_E_ARG, "weighted run on an unweighted graph");
+    if (p->weighted && !weighted) return set_error(GLP_E_ARG, "weighted run on an unweighted graph");
     // frequencies are packed as non-negative int32 scores
-    uint64_t wmax = 1;
-    if (p->weighted)
-        for (uint32_t w : g.weights) wmax = std::max<uint64_t>(wmax, w);
-    const uint64_t max_freq = wmax * (uint64_t)std::max(g.max_degree, 1);
+    const uint64_t wmax = p->weighted ? std::max<uint64_t>(wmax_in, 1) : 1;
+    const uint64_t max_freq = wmax * (uint64_t)std::max(max_degree, 1);
     if (max_freq > (uint64_t)INT32_MAX)
         return set_error(GLP_E_RANGE, "largest frequency sum %llu exceeds int32", (unsigned long long)max_freq);
     if (p->method == GLP_LAYERED) {
@@ -163,7 +168,7 @@ int check_params(const HostCSR& g, const glp_params* p) {
         if (q < 1 || (q & (q - 1)) || pn < 0)
             return set_error(GLP_E_ARG, "layered gamma must be p/q with q a power of two and p >= 0");
         // score (q+p)*C - p*V must stay inside int32
-        if ((q + pn) * (int64_t)max_freq > INT32_MAX || pn * (int64_t)g.n > INT32_MAX)
+        if ((q + pn) * (int64_t)max_freq > INT32_MAX || pn * (int64_t)n > INT32_MAX)
             return set_error(GLP_E_RANGE, "layered scores exceed int32 for gamma=%lld/%lld", (long long)pn, (long long)q);
     }
     if (p->method == GLP_SLPA) {

@@ -235,23 +235,50 @@ int glp_batch_autotune(glp_batch* b, int32_t steps, double* best_ms);
 void glp_batch_free(glp_batch* b);
 
 /* ---- serving: continuous batching under an HBM budget ------------------ */
+/* No reference counterpart (the reference runs one graph per process,
+ * main.cu:18-123).  Classic and layered LP only; results equal a solo run. */
 typedef struct glp_job_stats {
     double submit_ms;    /* host clock at submit, relative to server start      */
-    double admit_ms;     /* when its memory was reserved and state uploaded     */
-    double done_ms;      /* when its last step completed                        */
+    double admit_ms;     /* when its memory was reserved against the budget     */
+    double done_ms;      /* when its labels were copied out                     */
     int64_t bytes;       /* device footprint admitted against the budget        */
-    int32_t steps;
-    int32_t state;       /* 0 queued, 1 running, 2 done, 3 cancelled, <0 error  */
+    int32_t steps;       /* steps enqueued so far                               */
+    int32_t state;       /* 0 queued, 4 admitted, 1 running, 2 done,
+                            3 cancelled, <0 error                               */
+    double start_ms;     /* when its first step could be enqueued               */
 } glp_job_stats;
 int glp_server_create(uint64_t hbm_budget_bytes, int32_t max_slots,
                       glp_server** out);
-/* Queues a job (graph is copied into the server; the caller may free it).   */
+/* Resident graphs: on the device once, shared by every job that names them,
+ * charged against the budget until removed.  add_graph uploads a caller's
+ * graph; add_rmat builds glp_gen_rmat(scale, edge_factor, a, b, c, seed,
+ * degree_sort 0, wmax)'s graph directly in HBM (same CSR).                 */
+int glp_server_add_graph(glp_server* sv, const glp_graph* g, int32_t* graph_id);
+int glp_server_add_rmat(glp_server* sv, int32_t scale, int32_t edge_factor,
+                        double a, double b, double c, uint64_t seed,
+                        int32_t wmax, int32_t* graph_id);
+int glp_server_graph_info(glp_server* sv, int32_t graph_id, int32_t* n,
+                          int64_t* m, int32_t* max_degree, int64_t* bytes);
+/* Copies a resident graph's CSR out (offsets n+1, neighbours m, weights m;
+ * any may be NULL).                                                         */
+int glp_server_graph_csr(glp_server* sv, int32_t graph_id, int32_t* offsets,
+                         int32_t* neighbors, uint32_t* weights);
+/* The graph takes no new jobs and leaves once its running jobs end.         */
+int glp_server_remove_graph(glp_server* sv, int32_t graph_id);
+/* Queues a job on a resident graph.                                         */
+int glp_server_submit_resident(glp_server* sv, int32_t graph_id,
+                               const glp_params* p, int32_t steps,
+                               int64_t* job_id);
+/* Queues a job (graph is copied into the server; the caller may free it;
+ * it is uploaded when the job is admitted and freed when it ends).          */
 int glp_server_submit(glp_server* sv, const glp_graph* g, const glp_params* p,
                       int32_t steps, int64_t* job_id);
 int glp_server_poll(glp_server* sv, int64_t job_id, glp_job_stats* stats);
 /* Blocks until the job finishes, then copies its labels (n int32).          */
 int glp_server_wait(glp_server* sv, int64_t job_id, int32_t* labels_out,
                     glp_job_stats* stats);
+/* Forgets an ended job (its labels and stats); later calls on it fail.      */
+int glp_server_release(glp_server* sv, int64_t job_id);
 int glp_server_cancel(glp_server* sv, int64_t job_id);
 /* Drains all queued jobs and stops the request loop.                        */
 int glp_server_free(glp_server* sv);

@@ -263,6 +263,12 @@ def oracle_labels(g, steps, method, kw, weighted=False):
     return prev
 
 
+def oracle_labels_csr(off, nbr, w, steps, method, kw):
+    if method == G.CLASSIC:
+        return O.classic(off, nbr, steps, w=w)[-1]
+    return O.layered(off, nbr, steps, kw["gamma"][0], kw["gamma"][1], w=w)[0][-1]
+
+
 # small thresholds put the RMAT graph's high-degree vertices in the hub tier
 # (degree > 32) and the planted graphs' in the block tier
 HUB_TUNING = dict(pack_max_deg=8, wave_max_deg=16, block_max_deg=32, hub_chunk=256, part_keys=256)
@@ -332,6 +338,91 @@ def test_server_results_equal_alone_in_the_mix(tuned):
             alone, _ = G.run(g, s, params=p)
             assert np.array_equal(got, alone)
         sv.close()
+
+
+@pytest.mark.parametrize("scale,ef,seed,wmax", [(8, 8, 3, 0), (11, 16, 5, 8), (14, 32, 7, 8)])
+def test_server_device_rmat_equals_host_generator(scale, ef, seed, wmax):
+    """add_rmat builds Graph.rmat(..., degree_sort=False)'s CSR in HBM: the
+    same offsets, neighbours and weights."""
+    sv = G.Server(budget_bytes=1 << 30, max_slots=2)
+    gid = sv.add_rmat(scale, ef, seed=seed, wmax=wmax)
+    off, nbr, w = sv.graph_csr(gid, weights=wmax > 0)
+    host = G.Graph.rmat(scale, ef, seed=seed, wmax=wmax, degree_sort=False)
+    hoff, hnbr, hw = host.csr()
+    assert np.array_equal(off, hoff) and np.array_equal(nbr, hnbr)
+    if wmax:
+        assert np.array_equal(w, hw)
+    inf = sv.graph_info(gid)
+    assert inf["n"] == host.n and inf["m"] == host.m and inf["max_degree"] == host.info()["max_degree"]
+    sv.close()
+
+
+@pytest.mark.parametrize("tuned", [False, True])
+def test_server_resident_graphs_match_oracle(tuned):
+    """Jobs on device-generated resident graphs (classic and layered 1/4,
+    weighted), more jobs than slots and memory: every result equals the
+    oracle on the graph's CSR and the same job on a second, idle server."""
+    rng = np.random.default_rng(8)
+    with G.tuning(**(HUB_TUNING if tuned else {})):
+        sv = G.Server(budget_bytes=96 << 20, max_slots=3)
+        gids = [sv.add_rmat(int(rng.integers(8, 13)), int(rng.integers(8, 33)), seed=200 + i, wmax=8)
+                for i in range(4)]
+        csr = {g: sv.graph_csr(g, weights=True) for g in gids}
+        jobs = []
+        for i in range(14):
+            g = gids[int(rng.integers(0, len(gids)))]
+            layered = bool(rng.integers(0, 2))
+            p = G.make_params(method=G.LAYERED if layered else G.CLASSIC, gamma=(1, 4) if layered else (0, 1),
+                              weighted=True)
+            steps = int(rng.integers(1, 11))
+            jobs.append((sv.submit_resident(g, steps, params=p), g, p, steps))
+        solo = G.Server(budget_bytes=1 << 30, max_slots=1)
+        for jid, g, p, steps in jobs:
+            n = csr[g][0].shape[0] - 1
+            got, st = sv.wait(jid, n)
+            assert st.state == 2 and st.steps == steps
+            assert st.submit_ms <= st.admit_ms <= st.start_ms <= st.done_ms
+            off, nbr, w = csr[g]
+            kw = {"gamma": (p.gamma_num, p.gamma_den)}
+            want = oracle_labels_csr(off, nbr, w, steps, G.LAYERED if p.method == G.LAYERED else G.CLASSIC, kw)
+            assert np.array_equal(got, want), "job %d" % jid
+            alone, _ = solo.wait(solo.submit(G.Graph.from_csr(off, nbr, w), steps, params=p), n)
+            assert np.array_equal(got, alone)
+            sv.release(jid)
+            with pytest.raises(G.GLPError):
+                sv.poll(jid)
+        solo.close()
+        sv.close()
+
+
+def test_server_graph_removal_and_budget():
+    """A removed graph takes no new jobs but finishes its queued ones and
+    then returns its bytes; a job that cannot fit next to the resident
+    graphs fails instead of waiting, and fits once the graph is removed."""
+    host = G.Graph.rmat(12, 16, seed=1, degree_sort=False)
+    off, nbr, _ = host.csr()
+    probe = G.Server(budget_bytes=1 << 30, max_slots=1)
+    gb = probe.graph_info(probe.add_rmat(12, 16, seed=1))["bytes"]
+    probe.close()
+    # room for the resident graph and half as much again: a private copy of
+    # the same graph (its bytes plus the job's state) never fits beside it
+    sv = G.Server(budget_bytes=int(gb * 1.5), max_slots=2)
+    g1 = sv.add_rmat(12, 16, seed=1)
+    jid = sv.submit_resident(g1, 5)
+    jid2 = sv.submit(host, 3)
+    got, st = sv.wait(jid, host.n)
+    assert st.state == 2 and np.array_equal(got, O.classic(off, nbr, 5)[-1])
+    with pytest.raises(G.GLPError) as e:
+        sv.wait(jid2, host.n)
+    assert e.value.code == -8
+    sv.remove_graph(g1)
+    with pytest.raises(G.GLPError):
+        sv.submit_resident(g1, 1)
+    with pytest.raises(G.GLPError):
+        sv.graph_info(g1)
+    got, st = sv.wait(sv.submit(host, 3), host.n)
+    assert st.state == 2 and np.array_equal(got, O.classic(off, nbr, 3)[-1])
+    sv.close()
 
 
 def test_server_budget_and_cancel():
