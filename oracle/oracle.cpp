@@ -21,6 +21,10 @@
 //                        closed forms; hash KATs from the constants
 //   orc_table_step       capacity >= degree reduces to classic; hand traces
 //   orc_cpulp_run        mode 1 equals repeated orc_classic_step
+//   orc_classic_subset   equals orc_classic_step on every listed vertex
+//                        (random graphs, weighted and unweighted)
+//   orc_cpulp_layered_sample  timing only (the reference loop's live reads
+//                        race); its labels are checked to be neighbour labels
 
 #include "oracle.h"
 
@@ -389,6 +393,65 @@ extern "C" double orc_cpulp_sample(int32_t n, const int32_t* off, const int32_t*
                 max_label = label;
             }
         }
+        if (prelabels[u] != max_label) labels[u] = max_label;
+    }
+    auto t1 = std::chrono::system_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+extern "C" void orc_classic_subset(int32_t n, const int32_t* off, const int32_t* nbr, const uint32_t* w,
+                                   const int32_t* prev, const int32_t* verts, int64_t nverts, int32_t threads,
+                                   int32_t* next) {
+    (void)n;
+#pragma omp parallel num_threads(threads)
+    {
+        std::vector<std::pair<int32_t, uint32_t>> seen;
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t k = 0; k < nverts; ++k) {
+            const int32_t v = verts[k];
+            const int32_t b = off[v], e = off[v + 1];
+            if (b == e) {
+                next[k] = prev[v];
+                continue;
+            }
+            seen.clear();
+            for (int32_t j = b; j < e; ++j) seen.emplace_back(prev[nbr[j]], w ? w[j] : 1u);
+            std::sort(seen.begin(), seen.end());
+            Best<uint64_t> best;
+            for (size_t i = 0; i < seen.size();) {
+                size_t j = i;
+                uint64_t f = 0;
+                while (j < seen.size() && seen[j].first == seen[i].first) f += seen[j++].second;
+                best.offer(f, seen[i].first);
+                i = j;
+            }
+            next[k] = best.l;
+        }
+    }
+}
+
+extern "C" double orc_cpulp_layered_sample(int32_t n, const int32_t* off, const int32_t* nbr, int32_t* labels,
+                                           const int32_t* Vr, int32_t* Vw, const int32_t* verts, int64_t nverts,
+                                           int32_t threads, int32_t p, int32_t q) {
+    std::vector<int32_t> prelabels(labels, labels + n);
+    omp_set_schedule(omp_sched_static, 0);
+    auto t0 = std::chrono::system_clock::now();
+#pragma omp parallel for num_threads(threads) schedule(runtime)
+    for (int64_t k = 0; k < nverts; ++k) {
+        const int32_t u = verts[k];
+        std::map<int32_t, int> label_count;
+        int32_t max_label = prelabels[u];
+        int64_t max_score = INT64_MIN;
+        for (int32_t j = off[u]; j < off[u + 1]; ++j) {
+            const int32_t label = labels[nbr[j]];
+            const int c = ++label_count[label];
+            const int64_t s = (int64_t)(q + p) * c - (int64_t)p * Vr[label];
+            if (s > max_score) {
+                max_score = s;
+                max_label = label;
+            }
+        }
+        __atomic_fetch_add(&Vw[max_label], 1, __ATOMIC_RELAXED);
         if (prelabels[u] != max_label) labels[u] = max_label;
     }
     auto t1 = std::chrono::system_clock::now();

@@ -77,6 +77,22 @@ void orc_table_step(int32_t n, const int32_t* off, const int32_t* nbr, const uin
 double orc_cpulp_run(int32_t n, const int32_t* off, const int32_t* nbr, int32_t niter,
                      int32_t threads, int32_t mode, int32_t* labels_out);
 
+/* orc_classic_step for the vertices verts[0..nverts) only (next[k] is the
+ * label of verts[k]), threads in parallel: the check of single steps of a
+ * large graph on a sample of its vertices.  Neighbour labels are sorted per
+ * vertex instead of counted in a map; same argmax and tie rule. */
+void orc_classic_subset(int32_t n, const int32_t* off, const int32_t* nbr, const uint32_t* w,
+                        const int32_t* prev, const int32_t* verts, int64_t nverts, int32_t threads,
+                        int32_t* next);
+
+/* One step of tutorial/CPUlp.cuh:169-262 (layered_run) over verts[0..nverts),
+ * for timing: live labels[] read and updated in place, a std::map count per
+ * vertex, the running score (q+p)*count - p*Vr[label] with the strict '>'
+ * first maximum, and Vw[winner] incremented; gamma = p/q.  Returns seconds. */
+double orc_cpulp_layered_sample(int32_t n, const int32_t* off, const int32_t* nbr, int32_t* labels,
+                                const int32_t* Vr, int32_t* Vw, const int32_t* verts, int64_t nverts,
+                                int32_t threads, int32_t p, int32_t q);
+
 /* One verbatim CPUlp step (tutorial/CPUlp.cuh:47-81) restricted to the
  * vertices in verts[0..nverts): the bounded CPU-baseline sample.  labels is
  * read and updated in place as the reference does; balanced = 0 keeps the

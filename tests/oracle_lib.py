@@ -214,3 +214,31 @@ def cpulp_sample(off, nbr, labels, verts, threads, balanced=False):
     assert labels.dtype == np.int32 and labels.flags.c_contiguous
     return _o.orc_cpulp_sample(len(off) - 1, _i(off), _i(nbr), _i(labels), _i(verts), len(verts),
                                threads, int(balanced))
+
+
+_s("orc_classic_subset", None, ctypes.c_int32, _i32p, _i32p, _u32p, _i32p, _i32p, ctypes.c_int64,
+   ctypes.c_int32, _i32p)
+_s("orc_cpulp_layered_sample", ctypes.c_double, ctypes.c_int32, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p,
+   ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32)
+
+
+def classic_subset(off, nbr, prev, verts, w=None, threads=1):
+    """One classic step for the vertices `verts` only (labels in that order)."""
+    off, nbr, w = _csr(off, nbr, w)
+    prev = np.ascontiguousarray(prev, dtype=np.int32)
+    verts = np.ascontiguousarray(verts, dtype=np.int32)
+    out = np.zeros(max(len(verts), 1), dtype=np.int32)
+    _o.orc_classic_subset(len(off) - 1, _i(off), _i(nbr), _u(w), _i(prev), _i(verts), len(verts), threads,
+                          _i(out))
+    return out[:len(verts)]
+
+
+def cpulp_layered_sample(off, nbr, labels, Vr, Vw, verts, threads, p, q):
+    """One verbatim layered CPUlp step over `verts` (labels, Vw updated in
+    place); seconds."""
+    off, nbr, _ = _csr(off, nbr, None)
+    verts = np.ascontiguousarray(verts, dtype=np.int32)
+    for a in (labels, Vr, Vw):
+        assert a.dtype == np.int32 and a.flags.c_contiguous
+    return _o.orc_cpulp_layered_sample(len(off) - 1, _i(off), _i(nbr), _i(labels), _i(Vr), _i(Vw), _i(verts),
+                                       len(verts), threads, p, q)

@@ -103,6 +103,39 @@ def test_cpulp_sync_mode_matches_classic():
     assert out0.shape == (300,)
 
 
+def test_classic_subset_equals_full_step():
+    rng = np.random.default_rng(21)
+    for trial in range(6):
+        n = int(rng.integers(5, 300))
+        e = rng.integers(0, n, size=(int(rng.integers(0, 6 * n)), 2))
+        wts = rng.integers(1, 9, size=len(e)).tolist() if trial % 2 else None
+        off, nbr, w = O.csr_from_edges(n, [tuple(x) for x in e], weights=wts)
+        prev = rng.integers(0, 7, size=n).astype(np.int32)
+        full = O.classic_step(off, nbr, prev, w)
+        verts = rng.permutation(n)[: max(1, n // 2)].astype(np.int32)
+        assert np.array_equal(O.classic_subset(off, nbr, prev, verts, w=w, threads=3), full[verts])
+
+
+def test_cpulp_layered_sample_picks_neighbour_labels():
+    rng = np.random.default_rng(22)
+    off, nbr, _ = _random_graph(rng, 200, 800)
+    lab = np.arange(200, dtype=np.int32)
+    vr = np.ones(200, dtype=np.int32)
+    vw = np.zeros(200, dtype=np.int32)
+    before = lab.copy()
+    O.cpulp_layered_sample(off, nbr, lab, vr, vw, np.arange(200, dtype=np.int32), 1, 1, 4)
+    # one thread sweeps ids in order with live reads: a neighbour u < v
+    # already shows its new label, u > v still its old one
+    for v in range(200):
+        nb = nbr[off[v]:off[v + 1]]
+        if len(nb) == 0:
+            assert lab[v] == before[v]
+            continue
+        shown = [lab[u] if u < v else before[u] for u in nb]
+        assert lab[v] in shown
+    assert vw.sum() == 200
+
+
 # ---------------------------------------------------------------- layered ----
 
 def test_layered_hand_scores():
