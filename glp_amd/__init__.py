@@ -190,6 +190,7 @@ def lib():
 _sig("glp_autotune", ctypes.c_int, _vp, ctypes.POINTER(Params), ctypes.c_int32, ctypes.c_char_p,
      ctypes.POINTER(ctypes.c_double))
 _sig("glp_session_set_gamma", ctypes.c_int, _vp, ctypes.c_int32, ctypes.c_int32)
+_sig("glp_run_noalloc", ctypes.c_int, _vp, ctypes.POINTER(Params), ctypes.c_int32, _c_i32p, _c_i32p)
 _sig("glp_session_set_seeds", ctypes.c_int, _vp, ctypes.c_void_p, ctypes.c_int32)
 _sig("glp_run_seeded", ctypes.c_int, _vp, ctypes.POINTER(Params), ctypes.c_void_p, ctypes.c_int32,
      _c_i32p, _c_i32p, ctypes.POINTER(Timing))
@@ -552,6 +553,21 @@ def run(graph, steps, params=None, dump=False, seeds=None, **kw):
     if dump:
         return labels, dump_arr[:steps * n].reshape(steps, n), t.as_dict()
     return labels, t.as_dict()
+
+
+def run_noalloc(graph, steps, params=None, dump=False, **kw):
+    """run() for classic and layered LP through the allocation-free entry
+    point (all device state in the library's static array); labels, and the
+    per-step dump when dump=True."""
+    p = params if params is not None else make_params(**kw)
+    n = graph.n
+    labels = np.zeros(max(n, 1), dtype=np.int32)
+    dump_arr = np.zeros(max(steps * n, 1), dtype=np.int32) if dump else None
+    _check(_lib.glp_run_noalloc(graph.handle, ctypes.byref(p), steps, _i32(labels),
+                                _i32(dump_arr) if dump else None))
+    if dump:
+        return labels[:n], dump_arr[:steps * n].reshape(steps, n)
+    return labels[:n]
 
 
 class Batch:

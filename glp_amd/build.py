@@ -17,7 +17,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("GLP_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["kernels.hip", "counting.hip", "session.hip", "batch.hip", "server.hip", "plan.hip",
-               "autotune.hip", "gen_device.hip"]
+               "autotune.hip", "gen_device.hip", "noalloc.hip"]
 CPP_SOURCES = ["graph_io.cpp", "gen.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]

@@ -94,7 +94,8 @@ struct DeviceGraph {
     InverseIndex inv;
     double upload_ms = 0, plan_ms = 0;
     size_t bytes = 0;
-    bool owns_csr = true;  // false: off/nbr/w borrowed from another DeviceGraph
+    bool owns_csr = true;   // false: off/nbr/w borrowed from another DeviceGraph
+    bool owns_plan = true;  // false: the d_* plan arrays live in memory the graph does not own
     ~DeviceGraph();
 };
 int device_graph_create(const HostCSR& g, const Tuning& t, hipStream_t s, DeviceGraph** out);

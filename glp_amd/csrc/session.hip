@@ -27,6 +27,7 @@ DeviceGraph::~DeviceGraph() {
         (void)hipFree(nbr);
         (void)hipFree(w);
     }
+    if (!owns_plan) return;
     (void)hipFree(d_groups);
     (void)hipFree(d_half);
     (void)hipFree(d_wave);

@@ -218,6 +218,19 @@ int glp_session_slpa_post(glp_session* s, int32_t kmax, int32_t* dominant,
  * reference's free_gmem, prop/classicLprop.cuh:107-116).                    */
 int glp_session_free(glp_session* s);
 
+/* ---- allocation-free run ----------------------------------------------- */
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  glp_run for classic and layered LP (fused gather, no
+ * frontier, no timing record) with the graph's CSR, its tier plan and all
+ * LP state in a static device array of GLP_NOALLOC_BYTES per GPU, held from
+ * the library's load until the process ends; a graph whose CSR, plan and
+ * state do not fit fails with GLP_E_RANGE.  Copies use the null stream.
+ * Calls on one GPU share its array: the library serialises them.           */
+#define GLP_NOALLOC_BYTES (128u << 20)
+int glp_run_noalloc(const glp_graph* g, const glp_params* p, int32_t steps,
+                    int32_t* labels_out, int32_t* step_dump);
+
 /* ---- batched: many graphs in one persistent launch --------------------- */
 /* No single reference counterpart: the reference runs one graph per process
  * (main.cu:18-123).  All graphs use the same params; labels are identical to
