@@ -162,7 +162,9 @@ struct JobDesc {
     int32_t blocks[kLaunches];
 };
 
-enum class Method { Classic, Layered, Slpa, User, Seeded };
+// LayeredVol: layered LP with gamma > 0 scored in the table slots from the
+// (label, volume) records of StepArgs::LV (LayeredVolHooks).
+enum class Method { Classic, Layered, LayeredVol, Slpa, User, Seeded };
 
 // One-time kernel attribute set-up (LDS limits); call before any capture.
 void init_kernels();
@@ -196,6 +198,8 @@ int launch_frontier_gather(const StepArgs& a, int64_t m, int32_t* adj, const Inv
                            void* push, hipStream_t s);
 int launch_fill_i32(int32_t* p, int32_t value, int64_t count, hipStream_t s);
 int launch_iota(int32_t* p, int64_t count, hipStream_t s);
+// LV[u] = {L[u], V[L[u]]} for u < n (layered LP's records for LayeredVol).
+int launch_label_volumes(const int32_t* L, const int32_t* V, int2* LV, int32_t n, hipStream_t s);
 // V[l] += #{v < n : L[v] = l} (layered LP's volume of the written labels;
 // V zeroed first).
 int launch_volume(const int32_t* L, int32_t* V, int32_t n, hipStream_t s);

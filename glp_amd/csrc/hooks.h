@@ -61,6 +61,9 @@ struct StepArgs {
     int32_t lookahead;
     // seeded LP: seeds[v] != 0 marks a seed (its label stays v)
     const uint8_t* seeds;
+    // layered LP with gamma > 0: LV[u] = {L[u], Vr[L[u]]}, a neighbour's label
+    // and its label's volume in one 8-byte record (built before the tiers)
+    const int2* LV;
 };
 
 __host__ __device__ __forceinline__ uint32_t order_i32(int32_t s) {
@@ -151,6 +154,65 @@ struct LayeredHooks {
     }
 };
 
+// ---- layered LP with gamma > 0, scored in the table slot ---------------
+// The same scores as LayeredHooks, computed without a volume read per
+// candidate: each edge reads its neighbour's label together with that
+// label's volume (StepArgs::LV), and a table slot accumulates the score
+// itself, biased by 2^31 so that it stays a non-negative 32-bit sum: the
+// edge that creates the slot adds (q+p)f - pV(l) + 2^31, every later edge
+// of the label (q+p)f.  The sums only grow from a value >= 2^31 - p n > 0
+// to the final score + 2^31 < 2^32 (both bounds are checked on the host),
+// so a slot never carries into its key, and the slot word is the packed
+// score's high word as it stands.  Tables whose slots hold frequencies (the
+// hub tier's chunk tables, whose sums cross chunks) use label_score.
+struct LayeredVolHooks {
+    static constexpr bool kPick = false;
+    static constexpr bool kSkipIsolated = true;
+    static constexpr bool kFrontier = false;
+    static constexpr bool kScoreReadsVolume = true;  // for frequency-slot tables
+    static constexpr bool kSlotScore = true;
+    template <bool W, bool G>
+    __device__ static __forceinline__ uint32_t load_neighbor_vol(const StepArgs& a, int32_t vid, int32_t j,
+                                                                 uint32_t* freq, int32_t* vol) {
+        (void)vid;
+        *freq = W ? (a.w ? dev::ld_stream(&a.w[j]) : 1u) : 1u;
+        const int2 lv = a.LV[dev::ld_stream(&a.nbr[j])];
+        *vol = lv.y;
+        return (uint32_t)lv.x + 1u;
+    }
+    template <bool W, bool G>
+    __device__ static __forceinline__ uint32_t load_neighbor(const StepArgs& a, int32_t vid, int32_t j,
+                                                             uint32_t* freq) {
+        int32_t vol;
+        return load_neighbor_vol<W, G>(a, vid, j, freq, &vol);
+    }
+    __device__ static __forceinline__ uint32_t slot_more(const StepArgs& a, uint32_t f) {
+        return (uint32_t)a.score_mul * f;
+    }
+    __device__ static __forceinline__ uint32_t slot_first(const StepArgs& a, uint32_t f, int32_t vol) {
+        return slot_more(a, f) - (uint32_t)a.score_vmul * (uint32_t)vol + 0x80000000u;
+    }
+    __device__ static __forceinline__ uint64_t slot_score(uint32_t key, uint32_t slot) {
+        return ((uint64_t)slot << 32) | key;
+    }
+    __device__ static __forceinline__ int32_t label_volume(const StepArgs& a, uint32_t key) {
+        return LayeredHooks::label_volume(a, key);
+    }
+    __device__ static __forceinline__ uint64_t label_score_vol(const StepArgs& a, uint32_t key, uint32_t freq,
+                                                               int32_t vol) {
+        return LayeredHooks::label_score_vol(a, key, freq, vol);
+    }
+    __device__ static __forceinline__ uint64_t label_score(const StepArgs& a, int32_t vid, uint32_t key,
+                                                           uint32_t freq) {
+        return LayeredHooks::label_score(a, vid, key, freq);
+    }
+    __device__ static __forceinline__ void update_vertex(const StepArgs& a, int32_t vid, int32_t label,
+                                                         uint64_t score) {
+        (void)score;
+        a.Lw[vid] = label;
+    }
+};
+
 // ---- SLPA listener: classic counting over the speakers' labels; the
 // speaker choice is the pick kernel (slpa_speak).  prop/SLPAprop.cuh:337-398.
 struct SlpaHooks {
@@ -226,6 +288,44 @@ struct volume_of : std::false_type {};
 template <class H>
 struct volume_of<H, std::void_t<decltype(H::kScoreReadsVolume)>>
     : std::bool_constant<H::kScoreReadsVolume> {};
+
+// Hook sets with kSlotScore = true accumulate scores in table slots
+// (LayeredVolHooks); the others accumulate frequencies.
+template <class H, class = void>
+struct slot_score_of : std::false_type {};
+template <class H>
+struct slot_score_of<H, std::void_t<decltype(H::kSlotScore)>> : std::bool_constant<H::kSlotScore> {};
+
+// An edge's key, frequency and (slot-scoring hook sets) its label's volume.
+template <class H, bool W, bool G>
+__device__ __forceinline__ uint32_t load_edge(const StepArgs& a, int32_t v, int32_t j, uint32_t* fr,
+                                              int32_t* vol) {
+    if constexpr (slot_score_of<H>::value) {
+        return H::template load_neighbor_vol<W, G>(a, v, j, fr, vol);
+    } else {
+        *vol = 0;
+        return H::template load_neighbor<W, G>(a, v, j, fr);
+    }
+}
+// What an edge of frequency f adds to its label's slot when it creates the
+// slot (first) or finds it (more), and the packed score of a slot; S picks
+// the slot-score arithmetic (a hook set with kSlotScore in a table whose
+// slots hold scores) or plain frequency sums.
+template <class H, bool S>
+__device__ __forceinline__ uint32_t slot_first(const StepArgs& a, uint32_t f, int32_t vol) {
+    if constexpr (S) return H::slot_first(a, f, vol);
+    else { (void)a; (void)vol; return f; }
+}
+template <class H, bool S>
+__device__ __forceinline__ uint32_t slot_more(const StepArgs& a, uint32_t f) {
+    if constexpr (S) return H::slot_more(a, f);
+    else { (void)a; return f; }
+}
+template <class H, bool S>
+__device__ __forceinline__ uint64_t slot_value_score(const StepArgs& a, int32_t v, uint32_t key, uint32_t slot) {
+    if constexpr (S) { (void)a; (void)v; return H::slot_score(key, slot); }
+    else return H::label_score(a, v, key, slot);
+}
 
 // Hook sets without a kSeeded member have no fixed vertices.
 template <class H, class = void>

@@ -39,28 +39,32 @@ __device__ __forceinline__ T lds_load(const T* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Inserts (key, f) into an LDS table of (key << 32 | frequency sum) slots
-// and returns the slot, with kClaimed set when this insert created it.  A
-// key claims a free slot with its first frequency already in it; later ones
-// add to the low word (sums stay below 2^31, so they never carry into the
-// key).  The table always has a free slot.
+// Inserts key into an LDS table of (key << 32 | sum) slots and returns the
+// slot, with kClaimed set when this insert created it.  A key claims a free
+// slot with `first` already in it; a key that finds its slot adds `more` to
+// the low word (frequency sums stay below 2^31, and slot-scored sums below
+// 2^32 from a start above 0, so they never carry into the key).  The table
+// always has a free slot.
 constexpr uint32_t kClaimed = 0x80000000u;
 __device__ __forceinline__ uint32_t kc_insert(unsigned long long* T, uint32_t mask, uint32_t key,
-                                              uint32_t f) {
+                                              uint32_t first, uint32_t more) {
     uint32_t h = fmix32(key) & mask;
     const unsigned long long mine = (unsigned long long)key << 32;
     // the claim is tried without a read first: one LDS operation when the
     // slot is free, and no dearer than the read when it is not (measured
     // faster on diverse and on converged steps alike)
     for (;;) {
-        const unsigned long long cur = atomicCAS(&T[h], 0ull, mine | f);
+        const unsigned long long cur = atomicCAS(&T[h], 0ull, mine | first);
         if (cur == 0ull) return h | kClaimed;
         if ((uint32_t)(cur >> 32) == key) {
-            atomicAdd(&T[h], (unsigned long long)f);
+            atomicAdd(&T[h], (unsigned long long)more);
             return h;
         }
         h = (h + 1u) & mask;
     }
+}
+__device__ __forceinline__ uint32_t kc_insert(unsigned long long* T, uint32_t mask, uint32_t key, uint32_t f) {
+    return kc_insert(T, mask, key, f, f);
 }
 
 // Inserts one (key, f) per lane (key 0: nothing) into an LDS table; all 64
@@ -70,9 +74,9 @@ __device__ __forceinline__ uint32_t kc_insert(unsigned long long* T, uint32_t ma
 // merged into one insert of their summed frequency (sums wrap mod 2^32 as
 // the table's atomics do, so the counts are unchanged).
 constexpr int kPeels = 2;
-template <bool W>
-__device__ __forceinline__ void wave_insert(unsigned long long* T, uint32_t mask, uint32_t key,
-                                            uint32_t f) {
+template <class H, bool W, bool S>
+__device__ __forceinline__ void wave_insert(const StepArgs& a, unsigned long long* T, uint32_t mask, uint32_t key,
+                                            uint32_t f, int32_t vol) {
     const int lane = lane_id();
     uint32_t ik = key, iff = f;  // what this lane finally inserts
 #pragma unroll
@@ -91,7 +95,8 @@ __device__ __forceinline__ void wave_insert(unsigned long long* T, uint32_t mask
             key = 0u;
         }
     }
-    if (ik) kc_insert(T, mask, ik, iff);
+    // merged lanes show one label, so they share its volume
+    if (ik) kc_insert(T, mask, ik, slot_first<H, S>(a, iff, vol), slot_more<H, S>(a, iff));
 }
 
 // Best packed score over the occupied slots of an LDS table of `cap` slots,
@@ -101,12 +106,20 @@ __device__ __forceinline__ void wave_insert(unsigned long long* T, uint32_t mask
 // branch-free (an empty slot reads V[0]), so those loads are in flight
 // together instead of one after another.
 constexpr int kScanU = 8;
-template <class H>
+template <class H, bool S = false>
 __device__ __forceinline__ uint64_t table_best(const StepArgs& a, int32_t v, unsigned long long* T,
                                                uint32_t cap, uint32_t tid, uint32_t stride,
                                                bool clear) {
     uint64_t best = 0ull;
-    if constexpr (volume_of<H>::value) {
+    if constexpr (S) {
+        // slots hold the scores themselves
+        for (uint32_t i = tid; i < cap; i += stride) {
+            const unsigned long long x = T[i];
+            if (!x) continue;
+            best = max_u64(best, H::slot_score((uint32_t)(x >> 32), (uint32_t)x));
+            if (clear) T[i] = 0ull;
+        }
+    } else if constexpr (volume_of<H>::value) {
         for (uint32_t i0 = tid; i0 < cap; i0 += stride * kScanU) {
             unsigned long long x[kScanU];
             int32_t vol[kScanU];
@@ -140,13 +153,16 @@ __device__ __forceinline__ uint64_t table_best(const StepArgs& a, int32_t v, uns
 // thread first issues the loads of K edges (neighbour ids, then their
 // labels) and only then inserts them, so K gathers are in flight per thread
 // instead of one.  The loop is wave-uniform (wave_insert needs every lane).
-template <class H, bool W, bool G, int K, int STRIDE>
+// S: the slots accumulate scores (a slot-scoring hook set) instead of
+// frequencies.
+template <class H, bool W, bool G, int K, int STRIDE, bool S = false>
 __device__ __forceinline__ void count_edges(const StepArgs& a, int32_t v, int32_t b, int32_t e,
                                             int tid, unsigned long long* T, uint32_t mask) {
     static_assert(STRIDE % 64 == 0, "whole waves");
     const int lane = tid & 63;
     for (int32_t jw = b + (tid & ~63); jw < e; jw += STRIDE * K) {
         uint32_t key[K], fr[K];
+        int32_t vol[K];
         // branch-free: an index past the end is clamped to the last edge (a
         // valid address) and its key dropped afterwards, so the K neighbour
         // loads and then the K label loads issue back to back; a guarded load
@@ -154,13 +170,13 @@ __device__ __forceinline__ void count_edges(const StepArgs& a, int32_t v, int32_
 #pragma unroll
         for (int i = 0; i < K; ++i) {
             const int32_t j = jw + lane + i * STRIDE;
-            key[i] = H::template load_neighbor<W, G>(a, v, min(j, e - 1), &fr[i]);
+            key[i] = load_edge<H, W, G>(a, v, min(j, e - 1), &fr[i], &vol[i]);
         }
 #pragma unroll
         for (int i = 0; i < K; ++i)
             if (jw + lane + i * STRIDE >= e) key[i] = 0u;
 #pragma unroll
-        for (int i = 0; i < K; ++i) wave_insert<W>(T, mask, key[i], fr[i]);
+        for (int i = 0; i < K; ++i) wave_insert<H, W, S>(a, T, mask, key[i], fr[i], vol[i]);
     }
 }
 
@@ -258,6 +274,7 @@ constexpr uint32_t kPackSlots = 128;
 struct PackedGroup {
     int32_t v0, vc, base, ve, vb, nE, own, old;
     uint32_t key, fr;
+    int32_t vol;  // the label's volume (slot-scoring hook sets)
 };
 
 // Per-wave LDS of the packed count.
@@ -296,7 +313,7 @@ __device__ __forceinline__ void packed_load_fast(const StepArgs& a, int2 gr, Pac
     s.old = a.keep[s.v0 + li];
     s.ve = lane < s.vc ? end - s.base : 0;
     packed_segments(s, lane);
-    s.key = H::template load_neighbor<W, G>(a, s.v0 + s.own, s.base + min(lane, s.nE - 1), &s.fr);
+    s.key = load_edge<H, W, G>(a, s.v0 + s.own, s.base + min(lane, s.nE - 1), &s.fr, &s.vol);
     if (lane >= s.nE) {
         s.key = 0u;
         s.fr = 0u;
@@ -323,7 +340,8 @@ __device__ __forceinline__ bool packed_load_safe(const StepArgs& a, int2 gr, Pac
     packed_segments(s, lane);
     s.key = 0u;
     s.fr = 0u;
-    if (lane < s.nE) s.key = H::template load_neighbor<W, G>(a, s.v0 + s.own, s.base + lane, &s.fr);
+    s.vol = 0;
+    if (lane < s.nE) s.key = load_edge<H, W, G>(a, s.v0 + s.own, s.base + lane, &s.fr, &s.vol);
     return true;
 }
 
@@ -391,12 +409,19 @@ __device__ __forceinline__ uint32_t packed_finish(const StepArgs& a, const Packe
                 if (cur == k64) break;
                 h = (h + 1u) & (kPackSlots - 1u);
             }
-            atomicAdd(&t.cnt[h], W ? s.fr : 1u);
+            // a slot-scored count: every edge adds its share, the creator
+            // also the label's first-edge term (sums mod 2^32 are exact)
+            constexpr bool S = slot_score_of<H>::value;
+            const uint32_t f = W ? s.fr : 1u;
+            uint32_t add = slot_more<H, S>(a, f);
+            if (claimed) add += slot_first<H, S>(a, f, s.vol) - add;
+            atomicAdd(&t.cnt[h], add);
         }
         wave_lds_sync();
         // one score per distinct (owner, label): the slot's creator
         if (claimed) {
-            const uint64_t p = H::label_score(a, s.v0 + s.own, s.key, t.cnt[h]);
+            const uint64_t p =
+                slot_value_score<H, slot_score_of<H>::value>(a, s.v0 + s.own, s.key, t.cnt[h]);
             atomicMax(&t.best[s.own], (unsigned long long)p);
             t.key[h] = 0ull;
             t.cnt[h] = 0u;
@@ -480,6 +505,8 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restr
     int32_t v[NP], b[NP], e[NP], old[NP];
     bool run[NP];
     uint32_t key[NP][EPL], fr[NP][EPL];
+    int32_t vol[NP][EPL];
+    constexpr bool S_ = slot_score_of<H>::value;
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
         const int32_t idx = i0 + V * p + q;
@@ -501,7 +528,7 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restr
 #pragma unroll
         for (int i = 0; i < EPL; ++i) {
             const int32_t j = b[p] + ql + S * i;
-            key[p][i] = H::template load_neighbor<W, G>(a, v[p], run[p] ? min(j, e[p] - 1) : b[p], &fr[p][i]);
+            key[p][i] = load_edge<H, W, G>(a, v[p], run[p] ? min(j, e[p] - 1) : b[p], &fr[p][i], &vol[p][i]);
             if (!run[p] || j >= e[p]) key[p][i] = 0u;
         }
     unsigned long long* T = s_half[wv].slot[q];
@@ -601,7 +628,10 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restr
                     if (same && cnt >= 2u && lane != first) k = 0u;  // merged into the leader
                 }
                 sk[i] = k;
-                hs[i] = k ? kc_insert(T, kHalfCap - 1u, k, f) : 0u;
+                // a merged lane shows the leader's label, so the volumes agree
+                hs[i] = k ? kc_insert(T, kHalfCap - 1u, k, slot_first<H, S_>(a, f, vol[p][i]),
+                                      slot_more<H, S_>(a, f))
+                          : 0u;
             }
             wave_lds_sync();
             // one score per distinct label: the slot's creator
@@ -610,7 +640,7 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restr
             for (int i = 0; i < EPL; ++i)
                 if (sk[i] && (hs[i] & kClaimed)) {
                     unsigned long long* slot = &T[hs[i] & ~kClaimed];
-                    pm = max_u64(pm, H::label_score(a, v[p], sk[i], (uint32_t)*slot));
+                    pm = max_u64(pm, slot_value_score<H, S_>(a, v[p], sk[i], (uint32_t)*slot));
                     *slot = 0ull;
                 }
             // max over the group (xor distances below S stay in the group)
@@ -651,6 +681,8 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int4* __restr
     int32_t v[kWaveNV], b[kWaveNV], e[kWaveNV], old[kWaveNV];
     bool run[kWaveNV];
     uint32_t key[kWaveNV][EPL], fr[kWaveNV][EPL];
+    int32_t vol[kWaveNV][EPL];
+    constexpr bool S_ = slot_score_of<H>::value;
 #pragma unroll
     for (int k = 0; k < kWaveNV; ++k) {
         const int4 ent = verts[min(i0 + k, nv - 1)];
@@ -670,7 +702,7 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int4* __restr
 #pragma unroll
         for (int i = 0; i < EPL; ++i) {
             const int32_t j = b[k] + lane + 64 * i;
-            key[k][i] = H::template load_neighbor<W, G>(a, v[k], run[k] ? min(j, e[k] - 1) : b[k], &fr[k][i]);
+            key[k][i] = load_edge<H, W, G>(a, v[k], run[k] ? min(j, e[k] - 1) : b[k], &fr[k][i], &vol[k][i]);
             if (j >= e[k]) key[k][i] = 0u;
         }
     unsigned long long* T = s_tab[wv];
@@ -734,11 +766,13 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int4* __restr
             wave_lds_sync();
 #pragma unroll
             for (int i = 0; i < EPL; ++i) {
-                if (peel) wave_insert<W>(T, mask, key[k][i], fr[k][i]);
-                else if (key[k][i]) kc_insert(T, mask, key[k][i], fr[k][i]);
+                if (peel) wave_insert<H, W, S_>(a, T, mask, key[k][i], fr[k][i], vol[k][i]);
+                else if (key[k][i])
+                    kc_insert(T, mask, key[k][i], slot_first<H, S_>(a, fr[k][i], vol[k][i]),
+                              slot_more<H, S_>(a, fr[k][i]));
             }
             wave_lds_sync();
-            best = wave_max_u64(table_best<H>(a, v[k], T, cap, (uint32_t)lane, 64u, false));
+            best = wave_max_u64(table_best<H, S_>(a, v[k], T, cap, (uint32_t)lane, 64u, false));
             wave_lds_sync();  // the next vertex clears what this scan read
         }
         bool changed = false;
@@ -783,9 +817,10 @@ __device__ __forceinline__ void block_body(const StepArgs& a, const int32_t* __r
     const uint32_t mask = cap - 1u;
     for (uint32_t i = threadIdx.x; i < cap; i += NT) T[i] = 0ull;
     __syncthreads();
-    count_edges<H, W, G, K, NT>(a, v, b, e, threadIdx.x, T, mask);
+    constexpr bool S = slot_score_of<H>::value;
+    count_edges<H, W, G, K, NT, S>(a, v, b, e, threadIdx.x, T, mask);
     __syncthreads();
-    uint64_t best = table_best<H>(a, v, T, cap, threadIdx.x, NT, false);
+    uint64_t best = table_best<H, S>(a, v, T, cap, threadIdx.x, NT, false);
     best = block_max_u64<NT>(best, s_red);
     if (threadIdx.x < 64) {
         bool changed = false;
@@ -1759,6 +1794,14 @@ __global__ void k_seed_labels(int32_t* p, const uint8_t* __restrict__ seeds, int
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
         p[i] = seeds[i] ? (int32_t)i : -1;
 }
+__global__ void k_label_volumes(const int32_t* __restrict__ L, const int32_t* __restrict__ V,
+                                int2* __restrict__ LV, int32_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int32_t l = L[i];
+        LV[i] = make_int2(l, V[l]);
+    }
+}
 __global__ void k_iota(int32_t* p, int64_t count) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
@@ -2009,6 +2052,13 @@ void init_kernels() {
     GLP_SETLDS4(SlpaHooks)
     GLP_SETLDS4(UserHooks)
     GLP_SETLDS4(SeededHooks)
+    // slot-scored layered LP runs fused only (no gathered instantiation)
+    GLP_SETLDS(LayeredVolHooks, false, false)
+    GLP_SETLDS(LayeredVolHooks, true, false)
+    GLP_SETLDS_HUB(LayeredVolHooks, false, false)
+    GLP_SETLDS_HUB(LayeredVolHooks, true, false)
+    (void)hipFuncSetAttribute((const void*)k_hub_part<LayeredVolHooks>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              lim);
 #undef GLP_SETLDS4
 #undef GLP_SETLDS_HUB
 #undef GLP_SETLDS
@@ -2033,6 +2083,11 @@ int launch_step(Method method, bool weighted, bool gathered, const DeviceGraph& 
     switch (method) {
         case Method::Classic: return launch_hooks<ClassicHooks>(weighted, gathered, g, hs, a, s, ev);
         case Method::Layered: return launch_hooks<LayeredHooks>(weighted, gathered, g, hs, a, s, ev);
+        case Method::LayeredVol:
+            if (gathered || !a.LV)
+                return set_error(GLP_E_UNSUPPORTED, "slot-scored layered steps need the (label, volume) records");
+            return weighted ? launch_tiers<LayeredVolHooks, true, false>(g, hs, a, s, ev)
+                            : launch_tiers<LayeredVolHooks, false, false>(g, hs, a, s, ev);
         case Method::Slpa: return launch_hooks<SlpaHooks>(weighted, gathered, g, hs, a, s, ev);
         case Method::User: return launch_hooks<UserHooks>(weighted, gathered, g, hs, a, s, ev);
         case Method::Seeded: return launch_hooks<SeededHooks>(weighted, gathered, g, hs, a, s, ev);
@@ -2146,6 +2201,14 @@ int launch_fill_i32(int32_t* p, int32_t value, int64_t count, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_i32, dim3(grid), dim3(256), 0, s, p, value, count);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "fill launch: %s", hipGetErrorString(e));
+}
+
+int launch_label_volumes(const int32_t* L, const int32_t* V, int2* LV, int32_t n, hipStream_t s) {
+    if (n <= 0) return GLP_OK;
+    const unsigned grid = (unsigned)std::min<int64_t>(grid_for(n, 256), 256 * 64);
+    hipLaunchKernelGGL(k_label_volumes, dim3(grid), dim3(256), 0, s, L, V, LV, n);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "label-volume launch: %s", hipGetErrorString(e));
 }
 
 int launch_iota(int32_t* p, int64_t count, hipStream_t s) {

@@ -18,6 +18,7 @@ struct glp_session {
     hipStream_t stream = nullptr;
     int32_t* L[2] = {nullptr, nullptr};  // labels, double-buffered
     int32_t* V[2] = {nullptr, nullptr};  // layered volume histogram, double-buffered
+    int2* LV = nullptr;                  // layered, gamma > 0: {label, its volume} per vertex
     int32_t* shown = nullptr;            // PickLabel output (SLPA speakers, user hooks)
     int32_t* adj = nullptr;              // pre-gathered neighbour labels (gather mode)
     int32_t* adj_alt = nullptr;          // the second adj of the two-step lookahead

@@ -203,6 +203,7 @@ glp_session::~glp_session() {
         (void)hipFree(L[i]);
         (void)hipFree(V[i]);
     }
+    (void)hipFree(LV);
     (void)hipFree(shown);
     (void)hipFree(adj);
     (void)hipFree(adj_alt);
@@ -239,6 +240,7 @@ size_t session_bytes(const HostCSR& g, const Plan& p, const glp_params& prm) {
     size_t n = (size_t)g.n;
     size_t b = 2 * n * 4;
     if (prm.method == GLP_LAYERED) b += 2 * n * 4;
+    if (prm.method == GLP_LAYERED && !prm.gather) b += n * 8;  // LV records
     if (prm.method == GLP_SLPA) b += n * 4 + (size_t)(prm.slpa_T + 1) * n * 4;
     if (prm.method == GLP_USER) b += n * 4;
     if (prm.gather) b += (size_t)g.m * 4;
@@ -282,6 +284,7 @@ int session_create(glp_graph* g, const glp_params* prm, glp_session** out) {
     if (prm->method == GLP_LAYERED) {
         GLP_HIP(hipMalloc(&s->V[0], nn * 4));
         GLP_HIP(hipMalloc(&s->V[1], nn * 4));
+        if (!prm->gather) GLP_HIP(hipMalloc(&s->LV, nn * 8));
     }
     if (prm->method == GLP_SLPA || prm->method == GLP_USER) GLP_HIP(hipMalloc(&s->shown, nn * 4));
     if (prm->method == GLP_SLPA)
@@ -454,6 +457,14 @@ int enqueue_step(glp_session* s, hipEvent_t* ev) {
     int rc;
     if (s->method == Method::Layered && (rc = launch_fill_i32(s->V[s->cur ^ 1], 0, s->n, s->stream)))
         return rc;
+    // layered with gamma > 0: each neighbour's label travels with its
+    // label's volume, and the tables accumulate scores (LayeredVolHooks)
+    Method method = s->method;
+    if (s->method == Method::Layered && s->LV && s->p.gamma_num > 0) {
+        method = Method::LayeredVol;
+        a.LV = s->LV;
+        if ((rc = launch_label_volumes(a.L, a.Vr, s->LV, s->n, s->stream))) return rc;
+    }
     // frontier flags for the next step start clear (they held the last step's)
     if (s->fctl && (rc = launch_zero_bytes(s->act[s->cur ^ 1], s->act_bytes, s->stream))) return rc;
     // this step's changed-vertex bits start clear (they held two steps back)
@@ -471,8 +482,7 @@ int enqueue_step(glp_session* s, hipEvent_t* ev) {
         if (rc) return rc;
     }
     if (ev) (void)hipEventRecord(ev[2 * GLP_TIER_GATHER + 1], s->stream);
-    if ((rc = launch_step(s->method, s->p.weighted != 0, s->p.gather != 0, *s->g->dev, s->hs, a,
-                          s->stream, ev)))
+    if ((rc = launch_step(method, s->p.weighted != 0, s->p.gather != 0, *s->g->dev, s->hs, a, s->stream, ev)))
         return rc;
     if (s->method == Method::Layered && (rc = launch_volume(a.Lw, a.Vw, s->n, s->stream))) return rc;
     if ((rc = launch_end_step(s->d_acc, s->d_ring, s->d_ctr, kChangeRing, s->stream, s->fctl))) return rc;

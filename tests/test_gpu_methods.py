@@ -57,6 +57,67 @@ def test_layered_hub_path():
     assert_steps_equal(dump, O.layered(off, nbr, 4, 1, 1)[0], "layered hubs")
 
 
+def _layered_switch_oracle(off, nbr, plan, w=None):
+    """Layered steps with gamma changing between them: plan = [(steps, p, q)]."""
+    n = len(off) - 1
+    L = np.arange(n, dtype=np.int32)
+    V = np.ones(n, dtype=np.int32)
+    offc, nbrc, wc = O._csr(off, nbr, w)
+    out = []
+    for steps, p, q in plan:
+        for _ in range(steps):
+            nl = np.zeros(max(n, 1), dtype=np.int32)
+            nv = np.zeros(max(n, 1), dtype=np.int32)
+            O._o.orc_layered_step(n, O._i(offc), O._i(nbrc), O._u(wc), p, q, O._i(L), O._i(V), O._i(nl),
+                                  O._i(nv))
+            L, V = nl[:n], nv[:n]
+            out.append(L)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("gamma", [(1, 4), (3, 2)])
+def test_layered_slot_scores_and_volume_reads_agree(rmat14, gamma):
+    """gamma > 0: the fused steps score in the table slots from (label,
+    volume) records; the gathered form reads each candidate's volume.  Both
+    equal the oracle at every step."""
+    g, (off, nbr, _) = rmat14
+    want = O.layered(off, nbr, 5, gamma[0], gamma[1])[0]
+    _, dump, _ = G.run(g, 5, dump=True, method=G.LAYERED, gamma=gamma)
+    assert_steps_equal(dump, want, "slot-scored")
+    _, dump_g, _ = G.run(g, 5, dump=True, method=G.LAYERED, gamma=gamma, gather=True)
+    assert_steps_equal(dump_g, want, "gathered")
+
+
+def test_layered_slot_scores_every_tier_weighted():
+    """Tiny thresholds put vertices in every tier (hub chunks count
+    frequencies, every other table scores); weights up to 8."""
+    with G.tuning(pack_max_deg=8, half_max_deg=12, wave_max_deg=16, block_max_deg=32, hub_chunk=256,
+                  part_keys=256):
+        g = G.Graph.rmat(13, 16, seed=35, wmax=8)
+        off, nbr, w = g.csr()
+        _, dump, t = G.run(g, 4, dump=True, method=G.LAYERED, gamma=(1, 2), weighted=True)
+    assert min(t["tier_vertices"][k] for k in ("packed", "wave", "block", "hub")) > 0
+    assert_steps_equal(dump, O.layered(off, nbr, 4, 1, 2, w=w)[0], "slot-scored tiers")
+
+
+def test_layered_set_gamma_switches_scoring(rmat14):
+    """A session that starts at gamma 0 (frequency tables) and moves to 1/4
+    (slot scores) and back matches the oracle run with the same schedule."""
+    g, (off, nbr, _) = rmat14
+    want = _layered_switch_oracle(off, nbr, [(2, 0, 1), (3, 1, 4), (2, 0, 1)])
+    with G.Session(g, method=G.LAYERED, gamma=(0, 1)) as s:
+        s.step(2)
+        s.sync()
+        got = [s.labels()]
+        s.set_gamma(1, 4)
+        s.step(3)
+        got.append(s.labels())
+        s.set_gamma(0, 1)
+        s.step(2)
+        got.append(s.labels())
+    assert np.array_equal(got[0], want[1]) and np.array_equal(got[1], want[4]) and np.array_equal(got[2], want[6])
+
+
 def test_layered_range_check():
     g = G.Graph.rmat(10, 8, seed=1)
     with pytest.raises(G.GLPError):
