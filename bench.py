@@ -46,7 +46,7 @@ METRIC = "GTEPS per LP iteration (RMAT-25, 1 GPU); batched graphs/s vs plain; p5
 TIER_FAMILIES = {"packed": ["k_packed"], "wave": ["k_wave", "k_half"], "block": ["k_block"],
                  "hub": ["k_hub_agg"]}
 KERNEL_NAMES = {"packed": "k_packed", "wave": "k_wave + k_half", "block": "k_block (all degree classes)",
-                "hub": "k_hub_agg", "aux": "k_hub_scan+k_hub_scatter+k_hub_part+k_hub_final"}
+                "hub": "k_hub_agg", "aux": "k_hub_part+k_hub_final"}
 
 
 def log(*a):

@@ -107,17 +107,19 @@ int device_plan_upload(DeviceGraph* dg, hipStream_t s);
 size_t device_graph_bytes(const HostCSR& g, const Plan& p);
 
 // Per-session hub scratch: each chunk's aggregated (label+1, frequency)
-// pairs, the number of pairs per chunk, and per-hub winners (zero between
-// steps).
+// pairs grouped by bucket, the (bucket, chunk) runs they form, and per-hub
+// winners (zero between steps).
 struct HubScratch {
-    unsigned long long* pairs = nullptr;   // per chunk region: its distinct pairs
-    unsigned long long* sorted = nullptr;  // per hub: its pairs grouped by bucket
-    uint32_t* hist = nullptr;              // per hub: P x chunks counts, then offsets
-    uint32_t* total = nullptr;             // per hub: number of pairs
-    uint32_t* chunk_cnt = nullptr;
+    unsigned long long* pairs = nullptr;  // per chunk region: its distinct pairs, by bucket
+    uint32_t* hist = nullptr;             // per hub: P x chunks run lengths (bucket-major)
+    uint32_t* lstart = nullptr;           // per hub: P x chunks run starts in the chunk regions
     unsigned long long* best = nullptr;
     uint32_t* changed = nullptr;  // frontier: hub's label changed this step
 };
+// Device bytes of a plan's hub scratch (frontier: the changed flags too).
+inline size_t hub_scratch_bytes(const Plan& p, bool frontier) {
+    return (size_t)p.pair_slots * 8 + (size_t)p.hist_slots * 8 + p.hubs.size() * (frontier ? 12 : 8);
+}
 
 // Buckets of a hub of degree d (host and device agree): enough that a bucket
 // expects at most 3/4 part_keys distinct labels, at most kMaxBuckets.
@@ -148,8 +150,7 @@ struct HubArgs {
 // tier work lists.  blocks[] holds the workgroups it needs in each launch,
 // in the order LAUNCH_* below.
 enum { kLaunchPacked = 0, kLaunchHalf = 1, kLaunchWave = 2, kLaunchBlock0 = 3,
-       kLaunchHubAgg = 3 + kMaxBlockClasses, kLaunchHubScan, kLaunchHubScatter, kLaunchHubPart,
-       kLaunchHubFinal, kLaunches };
+       kLaunchHubAgg = 3 + kMaxBlockClasses, kLaunchHubPart, kLaunchHubFinal, kLaunches };
 struct JobDesc {
     StepArgs a;
     const int2* groups;

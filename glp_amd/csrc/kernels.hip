@@ -842,18 +842,18 @@ __device__ __forceinline__ void block_body(const StepArgs& a, const int32_t* __r
 }
 
 // ------------------------------------------------------------------- hub ----
-// Hubs are counted without any HBM hash table, in five passes:
-//  k_hub_agg     one workgroup per chunk of `chunk` edges: gathers the labels,
-//                counts them in an LDS table, writes the chunk's distinct
-//                (label+1, frequency) pairs to its pair region, and a
-//                histogram of those pairs over the hub's P buckets (a pair's
-//                bucket is fmix32(label+1) * P >> 32)
-//  k_hub_scan    one workgroup per hub: exclusive scan of the histogram in
-//                bucket-major order, i.e. where each chunk's share of each
-//                bucket starts once the hub's pairs are grouped by bucket
-//  k_hub_scatter one workgroup per chunk: moves its pairs into bucket order
-//  k_hub_part    one workgroup per (hub, bucket): sums its bucket's pairs in an
-//                LDS table and keeps the best; if the bucket holds more labels
+// Hubs are counted without any HBM hash table, in three passes:
+//  k_hub_agg     one workgroup per chunk of `chunk` edges: gathers the labels
+//                and counts them in an LDS table, then writes the chunk's
+//                distinct (label+1, frequency) pairs to its own pair region
+//                grouped by the hub's P label-hash buckets (a pair's bucket is
+//                fmix32(label+1) * P >> 32): a histogram over the buckets, its
+//                exclusive scan in LDS, then each pair to its bucket's place.
+//                Per (bucket, chunk) it records the run: how many pairs and
+//                where they start in the chunk's region.
+//  k_hub_part    one workgroup per batch of (hub, bucket) parts: a part sums
+//                its bucket's runs, read from every chunk region, in an LDS
+//                table and keeps the best; if the bucket holds more labels
 //                than the table, it counts the lower half of its fmix32 range,
 //                then the rest, halving as needed (fmix32 is a bijection, so a
 //                range no wider than the table cannot overflow)
@@ -872,15 +872,16 @@ __device__ __forceinline__ int32_t wave_incl_sum(int32_t x) {
     }
     return x;
 }
-// Exclusive prefix of x over a workgroup of 256 threads; *total = the sum.
+// Exclusive prefix of x over a workgroup of NT threads; *total = the sum.
+template <int NT = 256>
 __device__ __forceinline__ int32_t block_excl_sum(int32_t x, int32_t* total) {
-    __shared__ int32_t s_w[4];
+    __shared__ int32_t s_w[NT / 64];
     const int lane = lane_id(), wv = threadIdx.x >> 6;
     const int32_t incl = wave_incl_sum(x);
     if (lane == 63) s_w[wv] = incl;
     __syncthreads();
     int32_t base = 0, all = 0;
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < NT / 64; ++i) {
         if (i < wv) base += s_w[i];
         all += s_w[i];
     }
@@ -915,7 +916,6 @@ __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h
     // dynamic LDS: table of next_pow2(2*chunk) keys and counts, then the
     // bucket histogram (at least P counters for every hub of the launch)
     extern __shared__ __align__(16) uint32_t s_dyn[];
-    __shared__ uint32_t s_npairs;
     const uint32_t cap_max = next_pow2(2u * (uint32_t)h.chunk);
     unsigned long long* T = reinterpret_cast<unsigned long long*>(s_dyn);  // cap_max slots
     uint32_t* s_hist = s_dyn + 2 * cap_max;
@@ -924,10 +924,10 @@ __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h
     const int32_t vb = a.off[v], ve = a.off[v + 1];
     const int32_t nch = (ve - vb + h.chunk - 1) / h.chunk;
     const uint32_t P = hub_buckets(ve - vb, h.part_keys);
-    uint32_t* hist = h.hs.hist + h.hist_base[hc.x];  // [bucket][chunk]
+    uint32_t* hist = h.hs.hist + h.hist_base[hc.x];      // [bucket][chunk]: run lengths
+    uint32_t* lstart = h.hs.lstart + h.hist_base[hc.x];  // [bucket][chunk]: run starts in the chunk region
     if (!is_active<H>(a, v) || is_fixed<H>(a, v)) {  // block-uniform: the hub keeps its label
         for (uint32_t q = threadIdx.x; q < P; q += NT) hist[(int64_t)q * nch + hc.y] = 0u;
-        if (threadIdx.x == 0) h.hs.chunk_cnt[blk] = 0u;
         return;
     }
     const int32_t b = vb + hc.y * h.chunk;
@@ -936,201 +936,136 @@ __device__ __forceinline__ void hub_agg_body(const StepArgs& a, const HubArgs& h
     const uint32_t mask = cap - 1u;
     for (uint32_t i = threadIdx.x; i < cap; i += NT) T[i] = 0ull;
     for (uint32_t q = threadIdx.x; q < P; q += NT) s_hist[q] = 0u;
-    if (threadIdx.x == 0) s_npairs = 0u;
     __syncthreads();
     count_edges<H, W, G, K, NT>(a, v, b, e, threadIdx.x, T, mask);
     __syncthreads();
-    unsigned long long* out = h.hs.pairs + h.base[hc.x] + (int64_t)hc.y * h.chunk;
-    // cap is a power of two >= 64, so each wave runs this loop whole; one
-    // LDS atomic per wave reserves its pair slots
+    // the chunk's distinct pairs per bucket
     for (uint32_t i = threadIdx.x; i < cap; i += NT) {
-        const unsigned long long x = T[i];  // (label key << 32 | count): the pair itself
+        const uint32_t k = (uint32_t)(T[i] >> 32);
+        if (k) atomicAdd(&s_hist[bucket_of(fmix32(k), P)], 1u);
+    }
+    __syncthreads();
+    // run lengths out; their exclusive scan in place (each thread a run of
+    // consecutive buckets) gives the runs' starts, kept as bucket cursors
+    const uint32_t per = (P + NT - 1) / NT;
+    const uint32_t q0 = min(threadIdx.x * per, P), q1 = min(q0 + per, P);
+    uint32_t run = 0u;
+    for (uint32_t q = q0; q < q1; ++q) {
+        const uint32_t c = s_hist[q];
+        hist[(int64_t)q * nch + hc.y] = c;
+        run += c;
+    }
+    int32_t total;
+    uint32_t at = (uint32_t)block_excl_sum<NT>((int32_t)run, &total);
+    for (uint32_t q = q0; q < q1; ++q) {
+        const uint32_t c = s_hist[q];
+        lstart[(int64_t)q * nch + hc.y] = at;
+        s_hist[q] = at;
+        at += c;
+    }
+    __syncthreads();
+    // each pair (label key << 32 | count: the pair itself) to its bucket's run
+    unsigned long long* out = h.hs.pairs + h.base[hc.x] + (int64_t)hc.y * h.chunk;
+    for (uint32_t i = threadIdx.x; i < cap; i += NT) {
+        const unsigned long long x = T[i];
         const uint32_t k = (uint32_t)(x >> 32);
-        const unsigned long long bal = __ballot(k != 0u);
-        if (bal == 0ull) continue;
-        uint32_t first = 0u;
-        if (lane_id() == 0) first = atomicAdd(&s_npairs, (uint32_t)__popcll(bal));
-        first = (uint32_t)__builtin_amdgcn_readfirstlane((int)first);
-        if (!k) continue;
-        const uint32_t slot = first + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
-                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-        out[slot] = x;
-        atomicAdd(&s_hist[bucket_of(fmix32(k), P)], 1u);
-    }
-    __syncthreads();
-    for (uint32_t q = threadIdx.x; q < P; q += NT) hist[(int64_t)q * nch + hc.y] = s_hist[q];
-    if (threadIdx.x == 0) h.hs.chunk_cnt[blk] = s_npairs;
-}
-
-// Sum of hist[lo, hi), R entries per round loaded branch-free (clamped).
-template <int R>
-__device__ __forceinline__ uint32_t hist_run_sum(const uint32_t* hist, int64_t lo, int64_t hi) {
-    uint32_t sum = 0u;
-    for (int64_t i = lo; i < hi; i += R) {
-        uint32_t x[R];
-#pragma unroll
-        for (int k = 0; k < R; ++k) x[k] = hist[min(i + k, hi - 1)];
-#pragma unroll
-        for (int k = 0; k < R; ++k) sum += i + k < hi ? x[k] : 0u;
-    }
-    return sum;
-}
-// Replaces hist[lo, hi) by its exclusive prefix sums starting at `run`.
-template <int R>
-__device__ __forceinline__ void hist_run_offsets(uint32_t* hist, int64_t lo, int64_t hi, uint32_t run) {
-    for (int64_t i = lo; i < hi; i += R) {
-        uint32_t x[R];
-#pragma unroll
-        for (int k = 0; k < R; ++k) x[k] = hist[min(i + k, hi - 1)];
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-            if (i + k < hi) {
-                hist[i + k] = run;
-                run += x[k];
-            }
-        }
+        if (k) out[atomicAdd(&s_hist[bucket_of(fmix32(k), P)], 1u)] = x;
     }
 }
 
-// In-place exclusive scan of one hub's P x nch histogram (bucket-major);
-// also records the hub's pair total.
-__device__ __forceinline__ void hub_scan_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
-    const int32_t v = h.hubs[blk];
-    // a hub the frontier skips this step wrote an all-zero histogram, whose
-    // offsets are all zero already (block-uniform)
-    if (a.act_r && a.fctl[0] == 0 && a.act_r[v] == 0) {
-        if (threadIdx.x == 0) h.hs.total[blk] = 0u;
-        return;
-    }
-    const int32_t d = a.off[v + 1] - a.off[v];
-    const int64_t nch = (d + h.chunk - 1) / h.chunk;
-    const int64_t total = (int64_t)hub_buckets(d, h.part_keys) * nch;
-    uint32_t* hist = h.hs.hist + h.hist_base[blk];
-    // each thread owns a contiguous run of entries
-    const int64_t per = (total + kHubThreads - 1) / kHubThreads;
-    const int64_t lo = min((int64_t)threadIdx.x * per, total);
-    const int64_t hi = min(lo + per, total);
-    // R entries per round, loaded branch-free (clamped) so they are in
-    // flight together: the largest hubs' runs are hundreds of entries long,
-    // most hubs' one or two (per is block-uniform)
-    uint32_t sum;
-    if (per <= 4) sum = hist_run_sum<4>(hist, lo, hi);
-    else if (per <= 16) sum = hist_run_sum<16>(hist, lo, hi);
-    else sum = hist_run_sum<64>(hist, lo, hi);
-    int32_t pairs;
-    const uint32_t run = (uint32_t)block_excl_sum((int32_t)sum, &pairs);
-    if (threadIdx.x == 0) h.hs.total[blk] = (uint32_t)pairs;
-    if (per <= 4) hist_run_offsets<4>(hist, lo, hi, run);
-    else if (per <= 16) hist_run_offsets<16>(hist, lo, hi, run);
-    else hist_run_offsets<64>(hist, lo, hi, run);
-}
-
-// Moves one chunk's pairs to their bucket positions in the hub's sorted area.
-__device__ __forceinline__ void hub_scatter_body(const StepArgs& a, const HubArgs& h, uint32_t blk) {
-    __shared__ uint32_t s_cur[kMaxBuckets];
-    const int2 hc = h.chunks[blk];
-    const uint32_t n = h.hs.chunk_cnt[blk];
-    if (n == 0u) return;  // block-uniform
-    const int32_t v = h.hubs[hc.x];
-    const int32_t d = a.off[v + 1] - a.off[v];
-    const int64_t nch = (d + h.chunk - 1) / h.chunk;
-    const uint32_t P = hub_buckets(d, h.part_keys);
-    const uint32_t* hist = h.hs.hist + h.hist_base[hc.x];
-    for (uint32_t q = threadIdx.x; q < P; q += kHubThreads) s_cur[q] = hist[(int64_t)q * nch + hc.y];
-    __syncthreads();
-    const unsigned long long* src = h.hs.pairs + h.base[hc.x] + (int64_t)hc.y * h.chunk;
-    unsigned long long* dst = h.hs.sorted + h.base[hc.x];
-    // kR pairs per thread loaded (clamped, branch-free) before any is placed,
-    // so their loads are in flight together
-    constexpr int kR = 8;
-    for (uint32_t i0 = threadIdx.x; i0 < n; i0 += kHubThreads * kR) {
-        unsigned long long pr[kR];
-#pragma unroll
-        for (int r = 0; r < kR; ++r) pr[r] = src[min(i0 + r * kHubThreads, n - 1u)];
-#pragma unroll
-        for (int r = 0; r < kR; ++r) {
-            if (i0 + r * kHubThreads >= n) break;
-            const uint32_t q = bucket_of(fmix32((uint32_t)(pr[r] >> 32)), P);
-            dst[atomicAdd(&s_cur[q], 1u)] = pr[r];
-        }
-    }
-}
-
-// Sums the pairs of pr[0..n) whose fmix32 image lies in [lo, hi) into the LDS
-// table.  Stops early (the caller halves the range) once more than `limit`
-// distinct keys arrived; the table has room for limit + kHubThreads.
-__device__ __forceinline__ void part_pass(const unsigned long long* pr, uint32_t n, uint64_t lo,
-                                          uint64_t hi, unsigned long long* T, uint32_t shift,
-                                          uint32_t limit, uint32_t* s_distinct, uint32_t* s_over) {
-    const uint32_t mask = (1u << (32 - shift)) - 1u;
-    constexpr int kU = 4;  // pairs loaded per thread before any is inserted
-    // the trip count is the wave's (first lane's), so the wave reduction
-    // below runs with every lane; lanes past n load a clamped pair and skip it
-    for (uint32_t w0 = threadIdx.x & ~63u; w0 < n; w0 += kHubThreads * kU) {
-        if (lds_load(s_over)) return;  // one LDS read for the wave: uniform
-        const uint32_t i0 = w0 + (uint32_t)lane_id();
-        unsigned long long xs[kU];
-        // clamped indices keep the loads branch-free (issued together)
-#pragma unroll
-        for (int u = 0; u < kU; ++u) xs[u] = pr[min(i0 + u * kHubThreads, n - 1u)];
-        uint32_t claimed = 0u;
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const uint32_t key = (uint32_t)(xs[u] >> 32);
-            const uint32_t hk = fmix32(key);
-            if (i0 + u * kHubThreads >= n || hk < lo || hk >= hi) continue;
-            // a pair is already a (key << 32 | count) slot value; most pairs
-            // of a bucket bring a new label while labels are diverse, so the
-            // claim is tried first
-            uint32_t s = (hk * 0x9e3779b1u) >> shift;
-            for (uint32_t probes = 0;; ++probes) {
-                if (probes > mask) {
-                    // every slot is taken (waves that have not yet seen the
-                    // flag overshoot the limit): give the range up
-                    atomicExch(s_over, 1u);
-                    break;
-                }
-                const unsigned long long cur = atomicCAS(&T[s], 0ull, xs[u]);
-                if (cur == 0ull) {
-                    ++claimed;
-                    break;
-                }
-                if ((uint32_t)(cur >> 32) == key) {
-                    atomicAdd(&T[s], xs[u] & 0xffffffffull);
-                    break;
-                }
-                s = (s + 1u) & mask;
-            }
-        }
-        // one atomic per wave on the distinct-label counter
-        const int32_t wave_claimed = __builtin_amdgcn_readfirstlane(
-            shfl_i32(wave_incl_sum((int32_t)claimed), 63));
-        if (lane_id() == 0 && wave_claimed &&
-            atomicAdd(s_distinct, (uint32_t)wave_claimed) + (uint32_t)wave_claimed > limit)
-            atomicExch(s_over, 1u);
-    }
-}
-
-// A (hub, bucket) part: the hub, its vertex, the bucket and its pair range in
-// the hub's bucket-sorted pairs.
+// A (hub, bucket) part: the hub, its vertex, the bucket, the hub's buckets
+// and chunks, and the bucket's pairs over all chunks.
 struct PartJob {
     int32_t hub, v;
-    uint32_t q, P, start, stop;
+    uint32_t q, P, nch, total;
 };
 
-__device__ __forceinline__ PartJob part_lookup(const StepArgs& a, const HubArgs& h, int32_t p) {
-    PartJob j;
-    const int2 hp = h.parts[p];
-    j.hub = hp.x;
-    j.v = h.hubs[hp.x];
-    const int32_t d = a.off[j.v + 1] - a.off[j.v];
-    const int64_t nch = (d + h.chunk - 1) / h.chunk;
-    j.P = hub_buckets(d, h.part_keys);
-    j.q = (uint32_t)hp.y;
-    const uint32_t* hist = h.hs.hist + h.hist_base[hp.x];
-    j.start = hist[(int64_t)j.q * nch];
-    j.stop = j.q + 1 < j.P ? hist[(int64_t)(j.q + 1) * nch] : h.hs.total[hp.x];
-    return j;
+// Sums the part's pairs whose fmix32 image lies in [lo, hi) into the LDS
+// table.  The pairs lie in one run per chunk region; the workgroup walks the
+// chunks 64 at a time (every wave holds the 64 runs' lengths and starts, one
+// per lane) and spreads each window's pairs over its lanes, a pair's run
+// found by a binary search over the lanes' inclusive sums.  Stops early (the
+// caller halves the range) once more than `limit` distinct keys arrived;
+// the table has room for limit + kHubThreads.
+__device__ __forceinline__ void part_pass(const HubArgs& h, const PartJob& j, uint64_t lo, uint64_t hi,
+                                          unsigned long long* T, uint32_t shift, uint32_t limit,
+                                          uint32_t* s_distinct, uint32_t* s_over) {
+    const uint32_t mask = (1u << (32 - shift)) - 1u;
+    constexpr int kU = 4;  // pairs loaded per thread before any is inserted
+    constexpr uint32_t kWaves = kHubThreads / 64;
+    const int lane = lane_id();
+    const uint32_t wv = threadIdx.x >> 6;
+    const int64_t hb = h.hist_base[j.hub] + (int64_t)j.q * j.nch;
+    const uint32_t* cnt = h.hs.hist + hb;
+    const uint32_t* st = h.hs.lstart + hb;
+    const unsigned long long* pr = h.hs.pairs + h.base[j.hub];
+    for (uint32_t c0 = 0; c0 < j.nch; c0 += 64) {  // every wave walks the same windows
+        const uint32_t c = c0 + (uint32_t)lane;
+        const uint32_t n_c = c < j.nch ? cnt[c] : 0u;
+        const uint32_t s_c = c < j.nch ? c * (uint32_t)h.chunk + st[c] : 0u;
+        const int32_t incl = wave_incl_sum((int32_t)n_c);
+        const uint32_t wtot = (uint32_t)__builtin_amdgcn_readfirstlane(shfl_i32(incl, 63));
+        // the trip count is wave-uniform, so the wave reductions below run
+        // with every lane; lanes past the window's pairs load a clamped pair
+        // and skip it
+        for (uint32_t f0 = wv * 64u; f0 < wtot; f0 += kWaves * 64u * kU) {
+            if (lds_load(s_over)) return;  // one LDS read for the wave: uniform
+            unsigned long long xs[kU];
+            bool in[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const uint32_t f = f0 + (uint32_t)u * kWaves * 64u + (uint32_t)lane;
+                in[u] = f < wtot;
+                const int32_t fc = (int32_t)min(f, wtot - 1u);
+                // owner: the first lane whose inclusive sum exceeds fc
+                int o = 0, top = 63;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    const int mid = (o + top) >> 1;
+                    if (shfl_i32(incl, mid) <= fc) o = mid + 1;
+                    else top = mid;
+                }
+                const int32_t oi = shfl_i32(incl, o);
+                const uint32_t on = shfl_u32(n_c, o), os = shfl_u32(s_c, o);
+                xs[u] = pr[os + (uint32_t)(fc - (oi - (int32_t)on))];
+            }
+            uint32_t claimed = 0u;
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const uint32_t key = (uint32_t)(xs[u] >> 32);
+                const uint32_t hk = fmix32(key);
+                if (!in[u] || hk < lo || hk >= hi) continue;
+                // a pair is already a (key << 32 | count) slot value; most pairs
+                // of a bucket bring a new label while labels are diverse, so the
+                // claim is tried first
+                uint32_t s = (hk * 0x9e3779b1u) >> shift;
+                for (uint32_t probes = 0;; ++probes) {
+                    if (probes > mask) {
+                        // every slot is taken (waves that have not yet seen the
+                        // flag overshoot the limit): give the range up
+                        atomicExch(s_over, 1u);
+                        break;
+                    }
+                    const unsigned long long cur = atomicCAS(&T[s], 0ull, xs[u]);
+                    if (cur == 0ull) {
+                        ++claimed;
+                        break;
+                    }
+                    if ((uint32_t)(cur >> 32) == key) {
+                        atomicAdd(&T[s], xs[u] & 0xffffffffull);
+                        break;
+                    }
+                    s = (s + 1u) & mask;
+                }
+            }
+            // one atomic per wave on the distinct-label counter
+            const int32_t wave_claimed = __builtin_amdgcn_readfirstlane(
+                shfl_i32(wave_incl_sum((int32_t)claimed), 63));
+            if (lane == 0 && wave_claimed &&
+                atomicAdd(s_distinct, (uint32_t)wave_claimed) + (uint32_t)wave_claimed > limit)
+                atomicExch(s_over, 1u);
+        }
+    }
 }
 
 // Counts one non-empty part in the LDS table at s_dyn, which is all zero on
@@ -1140,8 +1075,7 @@ __device__ __forceinline__ void hub_part_count(const StepArgs& a, const HubArgs&
                                                unsigned long long* s_red, uint32_t* s_distinct,
                                                uint32_t* s_over) {
     extern __shared__ __align__(16) uint32_t s_dyn[];
-    const unsigned long long* pr = h.hs.sorted + h.base[j.hub] + j.start;
-    const uint32_t n = j.stop - j.start;
+    const uint32_t n = j.total;
     const uint32_t cap_max = next_pow2(2u * (uint32_t)h.part_keys);
     // a bucket of n <= cap_max/2 pairs gets a table of >= 2n slots: it holds
     // at most n distinct labels, so it cannot overflow, and converged buckets
@@ -1160,7 +1094,7 @@ __device__ __forceinline__ void hub_part_count(const StepArgs& a, const HubArgs&
             *s_over = 0u;
         }
         __syncthreads();
-        part_pass(pr, n, lo, hi, T, shift, limit, s_distinct, s_over);
+        part_pass(h, j, lo, hi, T, shift, limit, s_distinct, s_over);
         __syncthreads();
         if (*s_over != 0u) {
             // the pass stopped part-way: clear the table and retry the lower
@@ -1179,14 +1113,14 @@ __device__ __forceinline__ void hub_part_count(const StepArgs& a, const HubArgs&
     if (threadIdx.x == 0 && best) atomicMax(&h.hs.best[j.hub], (unsigned long long)best);
 }
 
-// One workgroup per batch of kPartBatch parts: the first kPartBatch threads
-// look up one part each, the non-empty ones are listed in LDS and counted
-// one after another.  Once labels converge nearly every bucket is empty, and
-// a launch of one workgroup per part would spend its time starting and
-// ending them; while labels are diverse every bucket is full, and a long
-// batch would count its parts in series on one CU.  16 parts per workgroup
-// measured best over both (hub_part on RMAT-25: diverse steps 2.8 -> 1.8 ms
-// against 256 parts, converged 0.11 -> 0.085 ms).
+// One workgroup per batch of kPartBatch parts: each wave sums the run
+// lengths of a quarter of them, the non-empty ones are listed in LDS and
+// counted one after another.  Once labels converge nearly every bucket is
+// empty, and a launch of one workgroup per part would spend its time
+// starting and ending them; while labels are diverse every bucket is full,
+// and a long batch would count its parts in series on one CU.  16 parts per
+// workgroup measured best over both (hub_part on RMAT-25: diverse steps 2.8
+// -> 1.8 ms against 256 parts, converged 0.11 -> 0.085 ms).
 constexpr int kPartBatch = 16;
 
 template <class H>
@@ -1198,10 +1132,20 @@ __device__ __forceinline__ void hub_part_batch_body(const StepArgs& a, const Hub
     __shared__ uint32_t s_n, s_distinct, s_over;
     if (threadIdx.x == 0) s_n = 0u;
     __syncthreads();
-    const int32_t p = (int32_t)(blk * kPartBatch + threadIdx.x);
-    if (threadIdx.x < kPartBatch && p < h.nparts) {
-        const PartJob j = part_lookup(a, h, p);
-        if (j.start != j.stop) s_jobs[atomicAdd(&s_n, 1u)] = j;
+    const int lane = lane_id();
+    for (int k = threadIdx.x >> 6; k < kPartBatch; k += kHubThreads / 64) {
+        const int32_t p = (int32_t)blk * kPartBatch + k;
+        if (p >= h.nparts) break;  // wave-uniform
+        const int2 hp = h.parts[p];
+        const int32_t v = h.hubs[hp.x];
+        const int32_t d = a.off[v + 1] - a.off[v];
+        const uint32_t nch = (uint32_t)((d + h.chunk - 1) / h.chunk);
+        const uint32_t* cnt = h.hs.hist + h.hist_base[hp.x] + (int64_t)hp.y * nch;
+        uint32_t sum = 0u;
+        for (uint32_t c = (uint32_t)lane; c < nch; c += 64) sum += cnt[c];
+        sum = wave_sum_u32(sum);
+        if (lane == 0 && sum)
+            s_jobs[atomicAdd(&s_n, 1u)] = PartJob{hp.x, v, (uint32_t)hp.y, hub_buckets(d, h.part_keys), nch, sum};
     }
     __syncthreads();
     const uint32_t n = s_n;
@@ -1275,12 +1219,6 @@ template <class H, bool W, bool G>
 __global__ __launch_bounds__(kHubAggThreads) void k_hub_agg(StepArgs a, HubArgs h) {
     hub_agg_body<H, W, G, kHubAggThreads, 4>(a, h, blockIdx.x);
 }
-__global__ __launch_bounds__(kHubThreads) void k_hub_scan(StepArgs a, HubArgs h) {
-    hub_scan_body(a, h, blockIdx.x);
-}
-__global__ __launch_bounds__(kHubThreads) void k_hub_scatter(StepArgs a, HubArgs h) {
-    hub_scatter_body(a, h, blockIdx.x);
-}
 template <class H>
 __global__ __launch_bounds__(kHubThreads) void k_hub_part(StepArgs a, HubArgs h) {
     hub_part_batch_body<H>(a, h, blockIdx.x);
@@ -1324,10 +1262,6 @@ __global__ __launch_bounds__(NT) void k_multi(const JobDesc* __restrict__ jobs,
         block_body<H, W, G, NT>(jd.a, jd.block[L - kLaunchBlock0], cap_max, blk);
     } else if constexpr (L == kLaunchHubAgg) {
         hub_agg_body<H, W, G>(jd.a, jd.hub, blk);
-    } else if constexpr (L == kLaunchHubScan) {
-        hub_scan_body(jd.a, jd.hub, blk);
-    } else if constexpr (L == kLaunchHubScatter) {
-        hub_scatter_body(jd.a, jd.hub, blk);
     } else if constexpr (L == kLaunchHubPart) {
         hub_part_batch_body<H>(jd.a, jd.hub, blk);
     } else {
@@ -1375,13 +1309,6 @@ static int launch_multi_t(const JobDesc* jobs, const int32_t* cum, int32_t nj, c
         hipLaunchKernelGGL((k_multi<H, W, false, kLaunchHubAgg, kHubThreads>), dim3(grid[kLaunchHubAgg]),
                            dim3(kHubThreads), lds, s, jobs, row(kLaunchHubAgg), nj, 0u);
     }
-    if (grid[kLaunchHubScan])
-        hipLaunchKernelGGL((k_multi<H, W, false, kLaunchHubScan, kHubThreads>), dim3(grid[kLaunchHubScan]),
-                           dim3(kHubThreads), 0, s, jobs, row(kLaunchHubScan), nj, 0u);
-    if (grid[kLaunchHubScatter])
-        hipLaunchKernelGGL((k_multi<H, W, false, kLaunchHubScatter, kHubThreads>),
-                           dim3(grid[kLaunchHubScatter]), dim3(kHubThreads), 0, s, jobs,
-                           row(kLaunchHubScatter), nj, 0u);
     if (grid[kLaunchHubPart]) {
         const size_t lds = (size_t)next_pow2(2u * (uint32_t)t.part_keys) * 2 * sizeof(uint32_t);
         hipLaunchKernelGGL((k_multi<H, W, false, kLaunchHubPart, kHubThreads>), dim3(grid[kLaunchHubPart]),
@@ -1410,8 +1337,6 @@ void job_blocks(const Plan& p, int32_t* blocks) {
     blocks[kLaunchWave] = (int32_t)((p.wave.size() + 4 * kWaveNV - 1) / (4 * kWaveNV));
     for (int c = 0; c < p.nclasses; ++c) blocks[kLaunchBlock0 + c] = (int32_t)p.block[c].size();
     blocks[kLaunchHubAgg] = (int32_t)p.hub_chunks.size();
-    blocks[kLaunchHubScan] = (int32_t)p.hubs.size();
-    blocks[kLaunchHubScatter] = (int32_t)p.hub_chunks.size();
     blocks[kLaunchHubPart] = (int32_t)((p.hub_parts.size() + kPartBatch - 1) / kPartBatch);
     blocks[kLaunchHubFinal] = (int32_t)((p.hubs.size() + 255) / 256);
 }
@@ -1988,8 +1913,6 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
                            hub_agg_lds(h.chunk, p.hub_max_buckets), s, a, h);
         mark(GLP_TIER_HUB, 1);
         mark(GLP_TIER_AUX, 0);
-        hipLaunchKernelGGL(k_hub_scan, dim3(h.nhubs), dim3(kHubThreads), 0, s, a, h);
-        hipLaunchKernelGGL(k_hub_scatter, dim3(nc), dim3(kHubThreads), 0, s, a, h);
         hipLaunchKernelGGL((k_hub_part<H>), dim3((np + kPartBatch - 1) / kPartBatch), dim3(kHubThreads),
                            lds_part, s, a, h);
         hipLaunchKernelGGL((k_hub_final<H>), dim3(grid_for(h.nhubs, 256)), dim3(256), 0, s, a, h);

@@ -74,10 +74,8 @@ size_t place(Arena& ar, const HostCSR& g, const Plan& p, bool layered, Layout* l
     lay->hs = HubScratch();
     if (!p.hubs.empty()) {
         lay->hs.pairs = ar.take<unsigned long long>((size_t)p.pair_slots);
-        lay->hs.sorted = ar.take<unsigned long long>((size_t)p.pair_slots);
         lay->hs.hist = ar.take<uint32_t>((size_t)p.hist_slots);
-        lay->hs.total = ar.take<uint32_t>(p.hubs.size());
-        lay->hs.chunk_cnt = ar.take<uint32_t>(p.hub_chunks.size());
+        lay->hs.lstart = ar.take<uint32_t>((size_t)p.hist_slots);
         lay->hs.best = ar.take<unsigned long long>(p.hubs.size());
     }
     lay->changes = ar.take<unsigned long long>(kChangeSlots);

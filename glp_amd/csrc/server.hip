@@ -135,7 +135,7 @@ struct ServerJob {
 
 // Sub-allocation of one job's state in one pooled block.
 struct JobLayout {
-    size_t L[2], V[2], pairs, sorted, hist, total, chunk_cnt, best, changes, bytes;
+    size_t L[2], V[2], pairs, hist, lstart, best, changes, bytes;
 };
 
 static JobLayout job_layout(int32_t n, const Plan& p, bool layered) {
@@ -152,10 +152,8 @@ static JobLayout job_layout(int32_t n, const Plan& p, bool layered) {
     o.V[0] = layered ? take(nn * 4) : 0;
     o.V[1] = layered ? take(nn * 4) : 0;
     o.pairs = take((size_t)p.pair_slots * 8);
-    o.sorted = take((size_t)p.pair_slots * 8);
     o.hist = take((size_t)p.hist_slots * 4);
-    o.total = take(p.hubs.size() * 4);
-    o.chunk_cnt = take(p.hub_chunks.size() * 4);
+    o.lstart = take((size_t)p.hist_slots * 4);
     o.best = take(p.hubs.size() * 8);
     o.changes = take(8 * kChangeSlots);
     o.bytes = at;
@@ -277,10 +275,8 @@ int glp_server::start_job(ServerJob* j) {
     j->V[1] = layered ? reinterpret_cast<int32_t*>(base + lay.V[1]) : nullptr;
     if (!p.hubs.empty()) {
         j->hs.pairs = reinterpret_cast<unsigned long long*>(base + lay.pairs);
-        j->hs.sorted = reinterpret_cast<unsigned long long*>(base + lay.sorted);
         j->hs.hist = reinterpret_cast<uint32_t*>(base + lay.hist);
-        j->hs.total = reinterpret_cast<uint32_t*>(base + lay.total);
-        j->hs.chunk_cnt = reinterpret_cast<uint32_t*>(base + lay.chunk_cnt);
+        j->hs.lstart = reinterpret_cast<uint32_t*>(base + lay.lstart);
         j->hs.best = reinterpret_cast<unsigned long long*>(base + lay.best);
         if ((rc = launch_zero_bytes(j->hs.best, (int64_t)p.hubs.size() * 8, stream))) return rc;
     }

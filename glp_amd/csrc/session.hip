@@ -209,10 +209,8 @@ glp_session::~glp_session() {
     (void)hipFree(adj_alt);
     (void)hipFree(mem);
     (void)hipFree(hs.pairs);
-    (void)hipFree(hs.sorted);
     (void)hipFree(hs.hist);
-    (void)hipFree(hs.total);
-    (void)hipFree(hs.chunk_cnt);
+    (void)hipFree(hs.lstart);
     (void)hipFree(hs.changed);
     (void)hipFree(act[0]);
     (void)hipFree(act[1]);
@@ -247,8 +245,7 @@ size_t session_bytes(const HostCSR& g, const Plan& p, const glp_params& prm) {
     if (prm.frontier) b += 2 * (n + 4);
     if (prm.frontier && prm.gather) b += 2 * ((n + 31) / 32) * 4 + push_entry_bytes(g.n, g.m);
     if (prm.frontier == 2 && prm.gather) b += (size_t)g.m * 4;  // the lookahead's second adj
-    b += (size_t)p.pair_slots * 16 + (size_t)p.hist_slots * 4 + p.hub_chunks.size() * 4 +
-         p.hubs.size() * 12;
+    b += hub_scratch_bytes(p, prm.frontier != 0);
     b += 8 * kChangeSlots + 8 * kChangeRing + 4;
     return b;
 }
@@ -294,10 +291,8 @@ int session_create(glp_graph* g, const glp_params* prm, glp_session** out) {
         GLP_HIP(hipMalloc(&s->adj_alt, std::max<size_t>((size_t)g->csr.m, 1) * 4));
     if (!p.hubs.empty()) {
         GLP_HIP(hipMalloc(&s->hs.pairs, (size_t)p.pair_slots * 8));
-        GLP_HIP(hipMalloc(&s->hs.sorted, (size_t)p.pair_slots * 8));
         GLP_HIP(hipMalloc(&s->hs.hist, (size_t)p.hist_slots * 4));
-        GLP_HIP(hipMalloc(&s->hs.total, p.hubs.size() * 4));
-        GLP_HIP(hipMalloc(&s->hs.chunk_cnt, p.hub_chunks.size() * 4));
+        GLP_HIP(hipMalloc(&s->hs.lstart, (size_t)p.hist_slots * 4));
         GLP_HIP(hipMalloc(&s->hs.best, p.hubs.size() * 8));
         GLP_HIP(hipMemsetAsync(s->hs.best, 0, p.hubs.size() * 8, s->stream));
         if (prm->frontier) GLP_HIP(hipMalloc(&s->hs.changed, p.hubs.size() * 4));
