@@ -42,6 +42,7 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_rmat_s25_classic.txt")
 TRAFFIC_SOURCE = ("profiles/pmc_traffic_rmat_s25_classic.txt: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 "
                   "per step (copy-kernel calibration), divided by this run's kernel time")
+TRAFFIC_FILE_LAYERED = os.path.join(ROOT, "profiles", "pmc_traffic_rmat_s25_layered_g1_4.txt")
 METRIC = "GTEPS per LP iteration (RMAT-25, 1 GPU); batched graphs/s vs plain; p50/p99"
 TIER_FAMILIES = {"packed": ["k_packed"], "wave": ["k_wave", "k_half"], "block": ["k_block"],
                  "hub": ["k_hub_agg"]}
@@ -255,7 +256,15 @@ def layered_legs(args, G, g, n, m):
         ls.sync()
         lms = (time.perf_counter() - t0) * 1e3 / args.steps
         tiers = tiers_over(base, ls.timing(), args.steps)
-        _, roof = dominant_roofline(tiers, 12, 16)
+        dom, roof = dominant_roofline(tiers, 12, 16)
+        if roof is not None and (gn, gd) == (1, 4):
+            # rocprof FETCH_SIZE + WRITE_SIZE of the same kernels at gamma 1/4
+            t = RS.traffic_for(TRAFFIC_FILE_LAYERED, "rmat-s%d-ef%d-seed%d-layered-1/4" % (
+                args.scale, args.edge_factor, args.seed), "main", TIER_FAMILIES[dom])
+            if t is not None:
+                roof["traffic"] = int(t)
+                roof["traffic_GBs"] = round(t / (tiers[dom]["ms"] / 1e3) / 1e9, 1)
+                roof["traffic_source"] = "profiles/pmc_traffic_rmat_s25_layered_g1_4.txt (calibrated as the classic file)"
         out["%d/%d" % (gn, gd)] = {
             "ms_per_step": round(lms, 4), "GTEPS": round(m / (lms / 1e3) / 1e9, 3),
             "from_L0_10_steps_ms_per_step": round(l0_ms, 4),
@@ -640,7 +649,7 @@ def main():
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="tuning override (pack_max_deg, wave_max_deg, block_max_deg, hub_chunk, "
                          "part_keys, part_limit, block_threads); repeatable")
-    ap.add_argument("--workload", choices=["rmat", "batched", "served", "copy", "random-read"],
+    ap.add_argument("--workload", choices=["rmat", "batched", "served", "slpa", "copy", "random-read"],
                     default="rmat")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--graphs", type=int, default=1024)
@@ -673,6 +682,12 @@ def main():
         return
     if args.workload == "served":
         print(json.dumps(bench_served(args, G, O)), flush=True)
+        return
+    if args.workload == "slpa":
+        out = slpa_leg(args, G)
+        out.update({"metric": "GTEPS per SLPA step (LFR-style 2^20, T=20)", "value": out["GTEPS"],
+                    "unit": "GTEPS"})
+        print(json.dumps(out), flush=True)
         return
     if args.workload == "copy":
         print(json.dumps(bench_copy(args)), flush=True)
