@@ -276,6 +276,29 @@ def layered_legs(args, G, g, n, m):
     return out
 
 
+def counting_legs(args, G, g, m):
+    """Count-min (3 x 200), frequent and buffer (16 slots) on the same graph:
+    three steps from L0, timed (labels stay diverse: the dearest steps)."""
+    out = {}
+    for name, kw in (("countmin", {"method": G.COUNTMIN, "cm_rows": 3, "cm_width": 200}),
+                     ("frequent", {"method": G.FREQUENT, "table_cap": 16}),
+                     ("buffer", {"method": G.BUFFER, "table_cap": 16})):
+        s = G.Session(g, G.make_params(**kw))
+        s.step(1)
+        s.sync()
+        s.reset()
+        t0 = time.perf_counter()
+        s.step(3)
+        s.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / 3
+        out[name] = {"ms_per_step": round(ms, 3), "GTEPS": round(m / (ms / 1e3) / 1e9, 3),
+                     "steps": "1-3 from L0",
+                     "shape": {k: v for k, v in kw.items() if k != "method"}}
+        log("%s: %.3f ms/step" % (name, ms))
+        s.free()
+    return out
+
+
 def from_l0_and_parity(args, G, O, g, off, nbr, n, m):
     """The first ten classic steps from L0, timed, and the oracle's check of
     steps 1 and 10 on every hub (degree > the block tier's bound) and a
@@ -432,7 +455,8 @@ def bench_batched(args, G):
     b = G.Batch(graphs, method=G.CLASSIC)
     tuned_ms = b.autotune(3) if not args.no_autotune else None
     chosen = {k: G.tuning_get(k) for k in ("pack_max_deg", "half_max_deg", "half_lanes", "wave_max_deg",
-                                           "block_max_deg", "block_threads", "hub_chunk", "part_keys")}
+                                           "block_max_deg", "block_threads", "hub_chunk", "part_keys",
+                                           "fused_small")}
     log(("autotuned: %s" % chosen) if tuned_ms else "not autotuned")
     # plain: one session per graph, stepped one graph after another
     b.run(1, mode=2)
@@ -756,7 +780,8 @@ def main():
     frontier = frontier_leg(args, G, g, m, dense_labels) if args.frontier else None
     extra = None
     if not args.no_extra:
-        extra = {"layered": layered_legs(args, G, g, n, m), "slpa": slpa_leg(args, G)}
+        extra = {"layered": layered_legs(args, G, g, n, m), "slpa": slpa_leg(args, G),
+                 "counting": counting_legs(args, G, g, m)}
     gather = gather_leg(args, G, g, m, wl_key) if args.gather_steps > 0 else None
     cpu = cpu_baseline(O, G, args, off, nbr) if not args.no_cpu else None
     del off, nbr

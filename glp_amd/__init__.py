@@ -177,7 +177,7 @@ _sig("glp_tuning_set", ctypes.c_int, ctypes.c_char_p, ctypes.c_int32)
 _sig("glp_tuning_get", ctypes.c_int, ctypes.c_char_p, _c_i32p)
 
 TUNING_KEYS = ("pack_max_deg", "half_max_deg", "half_lanes", "wave_max_deg", "block_max_deg", "hub_chunk",
-               "part_keys", "part_limit", "block_threads")
+               "part_keys", "part_limit", "block_threads", "fused_small")
 
 
 def lib():

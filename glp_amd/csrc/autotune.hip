@@ -31,6 +31,7 @@ const Candidates kSpace[] = {
     {"block_threads", {256, 512, 1024}},
     {"hub_chunk", {1024, 2048, 4096}},
     {"part_keys", {1024, 2048, 4096}},
+    {"fused_small", {0, 1}},
 };
 
 }  // namespace
@@ -86,7 +87,8 @@ extern "C" int glp_autotune(glp_graph* g, const glp_params* p, int32_t steps, co
         return r;
     };
 
-    Tuning cur = default_tuning();
+    // from the graph's current tuning
+    Tuning cur = base->plan.tune;
     double best = 0;
     if ((rc = time_tuning(cur, &best)) != GLP_OK) return rc;
     if (best < 0) return set_error(GLP_E_ARG, "current tuning is invalid for this graph");
@@ -131,7 +133,7 @@ extern "C" int glp_autotune(glp_graph* g, const glp_params* p, int32_t steps, co
         std::ofstream out(out_path);
         if (!out) return set_error(GLP_E_IO, "cannot write '%s'", out_path);
         for (const char* k : {"pack_max_deg", "half_max_deg", "wave_max_deg", "block_max_deg", "hub_chunk", "part_keys",
-                              "part_limit", "block_threads"}) {
+                              "part_limit", "block_threads", "fused_small"}) {
             int32_t v = 0;
             tuning_get(cur, k, &v);
             out << k << "=" << v << "\n";

@@ -25,6 +25,15 @@
 // the order-dependent semantics the reference's concurrent tables only
 // approximate.  The winner is the largest (count, label) left in the table;
 // a vertex whose table ends empty keeps its label.
+//
+// Degree tiers.  Count-min: one wavefront per vertex up to
+// kCountminWaveMaxDeg edges, one 1024-thread workgroup per vertex above (the
+// sketch is shared by the workgroup).  Frequent and buffer: one lane per
+// vertex up to kTableLaneMaxDeg edges (the lane walks its table slot by
+// slot), one wavefront per vertex above, with table slot i held by lane i
+// and each edge applied by the whole wavefront in CSR order: the order the
+// semantics fix, so both forms give the same table.  The vertices above the
+// thresholds are listed once per session.
 
 #include "device.h"
 #include "device_prims.h"
@@ -72,17 +81,24 @@ __device__ __forceinline__ int cm_cells(uint32_t key, int rows, int W, int alias
     return 4;
 }
 
-template <bool W>
-__global__ __launch_bounds__(64) void k_countmin(StepArgs a, int32_t rows, int32_t width,
-                                                 int32_t aliased) {
+constexpr int32_t kCountminWaveMaxDeg = 1024;
+constexpr int32_t kTableLaneMaxDeg = 32;
+
+// NT = 64: vertex blockIdx.x, skipped when its degree is above
+// kCountminWaveMaxDeg; NT = 1024: vertex verts[blockIdx.x].
+template <bool W, int NT>
+__global__ __launch_bounds__(NT) void k_countmin(StepArgs a, const int32_t* __restrict__ verts, int32_t rows,
+                                                 int32_t width, int32_t aliased) {
     extern __shared__ uint32_t cells[];
+    __shared__ unsigned long long s_best[NT / 64];
     const int lane = lane_id();
-    const int32_t v = (int32_t)blockIdx.x;
+    const int32_t v = NT > 64 ? verts[blockIdx.x] : (int32_t)blockIdx.x;
     const int32_t b = a.off[v], e = a.off[v + 1];
+    if (NT == 64 && e - b > kCountminWaveMaxDeg) return;  // workgroup-uniform: the wide launch takes it
     const int32_t ncell = rows * width;
-    for (int32_t i = lane; i < ncell; i += 64) cells[i] = 0u;
+    for (int32_t i = threadIdx.x; i < ncell; i += NT) cells[i] = 0u;
     __syncthreads();
-    for (int32_t j = b + lane; j < e; j += 64) {
+    for (int32_t j = b + (int32_t)threadIdx.x; j < e; j += NT) {
         const uint32_t key = (uint32_t)a.L[a.nbr[j]] + 1u;
         const uint32_t f = W ? a.w[j] : 1u;
         uint32_t c[5];
@@ -91,7 +107,7 @@ __global__ __launch_bounds__(64) void k_countmin(StepArgs a, int32_t rows, int32
     }
     __syncthreads();
     uint64_t best = 0ull;
-    for (int32_t j = b + lane; j < e; j += 64) {
+    for (int32_t j = b + (int32_t)threadIdx.x; j < e; j += NT) {
         const uint32_t key = (uint32_t)a.L[a.nbr[j]] + 1u;
         uint32_t c[5];
         const int k = cm_cells(key, rows, width, aliased, c);
@@ -100,7 +116,13 @@ __global__ __launch_bounds__(64) void k_countmin(StepArgs a, int32_t rows, int32
         best = max_u64(best, pack_score((int32_t)min(est, 0x7fffffffu), key));
     }
     best = wave_max_u64(best);
-    if (lane == 0) {
+    if (NT > 64) {
+        if (lane == 0) s_best[threadIdx.x >> 6] = best;
+        __syncthreads();
+        best = lane < NT / 64 ? s_best[lane] : 0ull;
+        best = wave_max_u64(best);
+    }
+    if (threadIdx.x == 0) {
         const int32_t old = a.keep[v];
         const int32_t nl = best ? packed_label(best) : old;
         a.Lw[v] = nl;
@@ -108,8 +130,69 @@ __global__ __launch_bounds__(64) void k_countmin(StepArgs a, int32_t rows, int32
     }
 }
 
-// One lane per vertex; the lane's table lives in LDS (C key and C count
-// slots per lane).
+// One wavefront per vertex of `verts` (degrees above kTableLaneMaxDeg):
+// slot i of the vertex's table is lane i's (K, C) (lanes >= cap hold no
+// slot).  Edges are loaded 64 at a time and applied one after another in
+// CSR order; a table's keys are distinct, so at most one lane hits.
+template <bool W, bool BUF>
+__global__ __launch_bounds__(256) void k_table_wave(StepArgs a, const int32_t* __restrict__ verts, int32_t nv,
+                                                    int32_t cap) {
+    const int lane = lane_id();
+    const int32_t i = (int32_t)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (i >= nv) return;  // wave-uniform
+    const int32_t v = verts[i];
+    const int32_t b = a.off[v], e = a.off[v + 1];
+    const bool slot = lane < cap;
+    uint32_t K = 0u, C = 0u;
+    for (int32_t j0 = b; j0 < e; j0 += 64) {
+        const int32_t j = j0 + lane;
+        uint32_t key = 0u, f = 0u;
+        if (j < e) {
+            key = (uint32_t)a.L[a.nbr[j]] + 1u;
+            f = W ? a.w[j] : 1u;
+        }
+        const int32_t cnt = min(64, e - j0);
+        for (int32_t k = 0; k < cnt; ++k) {
+            const uint32_t kk = (uint32_t)__builtin_amdgcn_readlane((int)key, k);
+            const uint32_t ff = W ? (uint32_t)__builtin_amdgcn_readlane((int)f, k) : 1u;
+            if (__ballot(slot && K == kk)) {  // present: it gains the frequency
+                if (slot && K == kk) C += ff;
+                continue;
+            }
+            const uint64_t fr = __ballot(slot && K == 0u);
+            if (fr) {  // the first free slot takes it
+                if (lane == __builtin_ctzll(fr)) {
+                    K = kk;
+                    C = ff;
+                }
+                continue;
+            }
+            // full: every counter drops (buffer: by the frequency; frequent:
+            // by min(frequency, smallest counter)), emptied slots are freed
+            uint32_t d = ff;
+            if (!BUF) d = min(ff, wave_min_u32(slot ? C : 0xffffffffu));
+            if (slot) C = BUF ? (C > ff ? C - ff : 0u) : C - d;
+            const bool emptied = slot && C == 0u;
+            if (emptied) K = 0u;
+            if (!BUF && ff > d) {  // frequent: what is left enters the first freed slot
+                if (lane == __builtin_ctzll(__ballot(emptied))) {
+                    K = kk;
+                    C = ff - d;
+                }
+            }
+        }
+    }
+    const uint64_t best = wave_max_u64(slot && K ? pack_score((int32_t)min(C, 0x7fffffffu), K) : 0ull);
+    if (lane == 0) {
+        const int32_t old = a.keep[v];
+        const int32_t nl = best ? packed_label(best) : old;
+        a.Lw[v] = nl;
+        if (nl != old) atomicAdd(&a.changes[(uint32_t)i & (kChangeSlots - 1)], 1ull);
+    }
+}
+
+// One lane per vertex of degree <= kTableLaneMaxDeg; the lane's table lives
+// in LDS (C key and C count slots per lane).
 template <bool W, bool BUF>
 __global__ __launch_bounds__(64) void k_table(StepArgs a, int32_t cap) {
     extern __shared__ uint32_t tbl[];
@@ -118,7 +201,7 @@ __global__ __launch_bounds__(64) void k_table(StepArgs a, int32_t cap) {
     uint32_t* K = tbl + lane * 2 * cap;
     uint32_t* C = K + cap;
     bool changed = false;
-    if (v < a.n) {
+    if (v < a.n && a.off[v + 1] - a.off[v] <= kTableLaneMaxDeg) {
         for (int i = 0; i < cap; ++i) { K[i] = 0u; C[i] = 0u; }
         const int32_t b = a.off[v], e = a.off[v + 1];
         for (int32_t j = b; j < e; ++j) {
@@ -163,8 +246,29 @@ __global__ __launch_bounds__(64) void k_table(StepArgs a, int32_t cap) {
 // without the attribute; k_table takes at most 64 * 2 * 64 * 4 = 32 KB.
 void init_counting_kernels() {
     const int lim = 5 * 4096 * 4;
-    (void)hipFuncSetAttribute((const void*)k_countmin<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-    (void)hipFuncSetAttribute((const void*)k_countmin<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void*)k_countmin<false, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void*)k_countmin<true, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void*)k_countmin<false, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              lim);
+    (void)hipFuncSetAttribute((const void*)k_countmin<true, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              lim);
+}
+
+// The vertices of degree above `max_deg`, for the counting methods' wide
+// launches (built once per session).
+int counting_wide_list(glp_session* s) {
+    const HostCSR& g = s->g->csr;
+    const int32_t lim = s->p.method == GLP_COUNTMIN ? kCountminWaveMaxDeg : kTableLaneMaxDeg;
+    std::vector<int32_t> wide;
+    for (int32_t v = 0; v < g.n; ++v)
+        if (g.offsets[v + 1] - g.offsets[v] > lim) wide.push_back(v);
+    s->nwide = (int32_t)wide.size();
+    if (wide.empty()) return GLP_OK;
+    GLP_HIP(hipMalloc(&s->wide, wide.size() * 4));
+    GLP_HIP(hipMemcpyAsync(s->wide, wide.data(), wide.size() * 4, hipMemcpyHostToDevice, s->stream));
+    GLP_HIP(hipStreamSynchronize(s->stream));  // `wide` is released on return
+    s->bytes += wide.size() * 4;
+    return GLP_OK;
 }
 
 int enqueue_counting_step(glp_session* s) {
@@ -183,20 +287,41 @@ int enqueue_counting_step(glp_session* s) {
     if (s->n > 0) {
         if (s->p.method == GLP_COUNTMIN) {
             const size_t lds = (size_t)s->p.cm_rows * s->p.cm_width * 4;
-            if (w) hipLaunchKernelGGL(k_countmin<true>, dim3(s->n), dim3(64), lds, s->stream, a,
-                                      s->p.cm_rows, s->p.cm_width, s->p.cm_aliased);
-            else hipLaunchKernelGGL(k_countmin<false>, dim3(s->n), dim3(64), lds, s->stream, a,
-                                    s->p.cm_rows, s->p.cm_width, s->p.cm_aliased);
+            const int32_t r = s->p.cm_rows, wd = s->p.cm_width, al = s->p.cm_aliased;
+            if (w) hipLaunchKernelGGL((k_countmin<true, 64>), dim3(s->n), dim3(64), lds, s->stream, a,
+                                      nullptr, r, wd, al);
+            else hipLaunchKernelGGL((k_countmin<false, 64>), dim3(s->n), dim3(64), lds, s->stream, a,
+                                    nullptr, r, wd, al);
+            if (s->nwide > 0) {
+                if (w) hipLaunchKernelGGL((k_countmin<true, 1024>), dim3(s->nwide), dim3(1024), lds, s->stream,
+                                          a, s->wide, r, wd, al);
+                else hipLaunchKernelGGL((k_countmin<false, 1024>), dim3(s->nwide), dim3(1024), lds,
+                                        s->stream, a, s->wide, r, wd, al);
+            }
         } else {
             const size_t lds = (size_t)64 * 2 * s->p.table_cap * 4;
             const unsigned grid = (unsigned)((s->n + 63) / 64);
             const bool buf = s->p.method == GLP_BUFFER;
+            const int32_t cap = s->p.table_cap;
+            const unsigned wgrid = (unsigned)((s->nwide + 3) / 4);
             if (buf) {
-                if (w) hipLaunchKernelGGL((k_table<true, true>), dim3(grid), dim3(64), lds, s->stream, a, s->p.table_cap);
-                else hipLaunchKernelGGL((k_table<false, true>), dim3(grid), dim3(64), lds, s->stream, a, s->p.table_cap);
+                if (w) hipLaunchKernelGGL((k_table<true, true>), dim3(grid), dim3(64), lds, s->stream, a, cap);
+                else hipLaunchKernelGGL((k_table<false, true>), dim3(grid), dim3(64), lds, s->stream, a, cap);
+                if (s->nwide > 0) {
+                    if (w) hipLaunchKernelGGL((k_table_wave<true, true>), dim3(wgrid), dim3(256), 0, s->stream, a,
+                                              s->wide, s->nwide, cap);
+                    else hipLaunchKernelGGL((k_table_wave<false, true>), dim3(wgrid), dim3(256), 0, s->stream, a,
+                                            s->wide, s->nwide, cap);
+                }
             } else {
-                if (w) hipLaunchKernelGGL((k_table<true, false>), dim3(grid), dim3(64), lds, s->stream, a, s->p.table_cap);
-                else hipLaunchKernelGGL((k_table<false, false>), dim3(grid), dim3(64), lds, s->stream, a, s->p.table_cap);
+                if (w) hipLaunchKernelGGL((k_table<true, false>), dim3(grid), dim3(64), lds, s->stream, a, cap);
+                else hipLaunchKernelGGL((k_table<false, false>), dim3(grid), dim3(64), lds, s->stream, a, cap);
+                if (s->nwide > 0) {
+                    if (w) hipLaunchKernelGGL((k_table_wave<true, false>), dim3(wgrid), dim3(256), 0, s->stream,
+                                              a, s->wide, s->nwide, cap);
+                    else hipLaunchKernelGGL((k_table_wave<false, false>), dim3(wgrid), dim3(256), 0, s->stream,
+                                            a, s->wide, s->nwide, cap);
+                }
             }
         }
         hipError_t e = hipGetLastError();

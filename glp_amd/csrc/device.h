@@ -29,6 +29,9 @@ struct Tuning {
     int32_t part_keys = 2048;      // expected distinct labels per hub counting workgroup
     int32_t part_limit = 0;        // distinct labels before a hub range is halved (0: table/2)
     int32_t block_threads = 256;   // block-tier workgroup size (256: by table size, up to 1024)
+    // 1: the packed and half tiers run as one persistent launch whose waves
+    // take units from a counter; 0: one launch per tier and shape
+    int32_t fused_small = 0;
 };
 int tuning_set(Tuning& t, const char* key, int32_t value);  // GLP_E_ARG on unknown key
 int tuning_get(const Tuning& t, const char* key, int32_t* value);
@@ -115,6 +118,7 @@ struct HubScratch {
     uint32_t* lstart = nullptr;           // per hub: P x chunks run starts in the chunk regions
     unsigned long long* best = nullptr;
     uint32_t* changed = nullptr;  // frontier: hub's label changed this step
+    uint32_t* work = nullptr;     // the persistent small-tier launch's unit counter
 };
 // Device bytes of a plan's hub scratch (frontier: the changed flags too).
 inline size_t hub_scratch_bytes(const Plan& p, bool frontier) {

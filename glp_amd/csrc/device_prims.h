@@ -40,6 +40,12 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
     for (int d = 32; d >= 1; d >>= 1) x = max_u64(x, shfl_u64(x, lane ^ d));
     return x;
 }
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x = min(x, shfl_u32(x, lane ^ d));
+    return x;
+}
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
     const int lane = lane_id();
 #pragma unroll

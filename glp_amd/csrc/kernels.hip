@@ -448,16 +448,14 @@ __device__ __forceinline__ uint32_t packed_finish(const StepArgs& a, const Packe
     return (uint32_t)__popcll(__ballot(changed));
 }
 
-// ngroups: groups to run; groups [0, nge) have edges.
+// ngroups: groups to run; groups [0, nge) have edges.  The wave runs unit
+// `wunit` (groups [wunit NG, wunit NG + NG)) in its LDS area t.
 template <class H, bool W, bool G, int NG>
 __device__ __forceinline__ void packed_body(const StepArgs& a, const int2* __restrict__ groups,
-                                            int32_t ngroups, int32_t nge, uint32_t blk) {
-    __shared__ PackedLds s_pk[4];
+                                            int32_t ngroups, int32_t nge, uint32_t wunit, PackedLds& t) {
     // wave-uniform first group (readfirstlane keeps the group loads scalar)
-    const int32_t g0 =
-        __builtin_amdgcn_readfirstlane((int32_t)((blk * 256u + threadIdx.x) >> 6)) * NG;
+    const int32_t g0 = __builtin_amdgcn_readfirstlane((int32_t)wunit) * NG;
     if (g0 >= ngroups) return;
-    PackedLds& t = s_pk[threadIdx.x >> 6];
     bool cleared = false;  // the wave's table (wave-uniform)
     uint32_t nch = 0u;
     if (g0 + NG <= nge && !frontier_on<H>(a)) {
@@ -490,17 +488,17 @@ struct HalfLds {
     unsigned long long slot[64 / S][kHalfCap];
 };
 
-// EPL: edges loaded per lane (S * EPL >= the launch's largest degree)
+// EPL: edges loaded per lane (S * EPL >= the launch's largest degree).
+// The wave runs unit `wunit` (vertices [wunit V NP, ...)) in its LDS area.
 template <class H, bool W, bool G, int NP, int S = 32, int EPL = kHalfMaxDeg / S>
 __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restrict__ verts,
-                                          int32_t nv, uint32_t blk) {
+                                          int32_t nv, uint32_t wunit, HalfLds<S>& lds) {
     static_assert(S == 16 || S == 32, "16 or 32 lanes per vertex");
     constexpr int V = 64 / S;               // vertices per unit
-    __shared__ HalfLds<S> s_half[4];
-    const int lane = lane_id(), wv = threadIdx.x >> 6, q = lane / S, ql = lane % S;
+    const int lane = lane_id(), q = lane / S, ql = lane % S;
     const int g0 = q * S;  // the group's first lane
     const unsigned long long gm = ((1ull << S) - 1ull) << g0;  // the group's lanes
-    const int32_t i0 = ((int32_t)blk * 4 + wv) * (V * NP);
+    const int32_t i0 = (int32_t)wunit * (V * NP);
     if (i0 >= nv) return;  // wave-uniform
     int32_t v[NP], b[NP], e[NP], old[NP];
     bool run[NP];
@@ -531,7 +529,7 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restr
             key[p][i] = load_edge<H, W, G>(a, v[p], run[p] ? min(j, e[p] - 1) : b[p], &fr[p][i], &vol[p][i]);
             if (!run[p] || j >= e[p]) key[p][i] = 0u;
         }
-    unsigned long long* T = s_half[wv].slot[q];
+    unsigned long long* T = lds.slot[q];
     // the tables are cleared before their first use, and each unit leaves
     // them empty: a slot's creator clears it once it has scored it
     bool cleared = false;  // wave-uniform
@@ -671,12 +669,13 @@ __device__ __forceinline__ void half_body(const StepArgs& a, const int4* __restr
 constexpr int kWaveNV = 4;
 // EPL: edges loaded per lane (4 covers 256 edges; 2 serves vertices of at
 // most 128 edges without clamped duplicate loads)
+// The wave runs unit `wunit` (vertices [wunit kWaveNV, ...)) with its LDS
+// table T of kWaveCap (label key << 32 | count) slots.
 template <class H, bool W, bool G, int EPL = 4>
 __device__ __forceinline__ void wave_body(const StepArgs& a, const int4* __restrict__ verts,
-                                          int32_t nv, uint32_t blk) {
-    __shared__ unsigned long long s_tab[4][kWaveCap];  // (label key << 32 | count) slots
-    const int lane = lane_id(), wv = threadIdx.x >> 6;
-    const int32_t i0 = ((int32_t)blk * 4 + wv) * kWaveNV;
+                                          int32_t nv, uint32_t wunit, unsigned long long* T) {
+    const int lane = lane_id();
+    const int32_t i0 = (int32_t)wunit * kWaveNV;
     if (i0 >= nv) return;  // wave-uniform
     int32_t v[kWaveNV], b[kWaveNV], e[kWaveNV], old[kWaveNV];
     bool run[kWaveNV];
@@ -705,7 +704,6 @@ __device__ __forceinline__ void wave_body(const StepArgs& a, const int4* __restr
             key[k][i] = load_edge<H, W, G>(a, v[k], run[k] ? min(j, e[k] - 1) : b[k], &fr[k][i], &vol[k][i]);
             if (j >= e[k]) key[k][i] = 0u;
         }
-    unsigned long long* T = s_tab[wv];
     uint32_t nch = 0u;  // changed labels (lane 0)
 #pragma unroll
     for (int k = 0; k < kWaveNV; ++k) {
@@ -1195,20 +1193,78 @@ __device__ __forceinline__ void hub_mark_body(const StepArgs& a, const HubArgs& 
 
 // ------------------------------------------------------------- wrappers ----
 // Single job: workgroup index = blockIdx.x.
+
+// One persistent small-tier launch's work: the unit ranges, heaviest first
+// (half tier with 16 lanes per vertex and 4, then 2 edges per lane; packed
+// groups); end[] holds the cumulative unit counts.
+constexpr int kSmallNG = 4;  // packed groups per unit
+constexpr int kSmallNP = 2;  // half-tier quarter-wave units per unit (8 vertices)
+struct SmallTiers {
+    const int4* half_long;
+    const int4* half_short;
+    const int2* groups;
+    int32_t n_half_long, n_half_short, ngroups, nge;
+    uint32_t end[3];
+    uint32_t* counter;  // zero at launch; a wave takes kSmallGrab units per atomic
+};
+constexpr uint32_t kSmallGrab = 16;
+
+// A wave's unit in a launch of 256-thread workgroups.
+__device__ __forceinline__ uint32_t wave_unit(uint32_t blk) { return blk * 4u + (threadIdx.x >> 6); }
+
 template <class H, bool W, bool G, int NG>
 __global__ __launch_bounds__(256) void k_packed(StepArgs a, const int2* __restrict__ groups,
                                                 int32_t ngroups, int32_t nge) {
-    packed_body<H, W, G, NG>(a, groups, ngroups, nge, blockIdx.x);
+    __shared__ PackedLds s_pk[4];
+    packed_body<H, W, G, NG>(a, groups, ngroups, nge, wave_unit(blockIdx.x), s_pk[threadIdx.x >> 6]);
 }
 template <class H, bool W, bool G, int NP, int S, int EPL>
 __global__ __launch_bounds__(256) void k_half(StepArgs a, const int4* __restrict__ verts,
                                               int32_t nv) {
-    half_body<H, W, G, NP, S, EPL>(a, verts, nv, blockIdx.x);
+    __shared__ HalfLds<S> s_half[4];
+    half_body<H, W, G, NP, S, EPL>(a, verts, nv, wave_unit(blockIdx.x), s_half[threadIdx.x >> 6]);
 }
 template <class H, bool W, bool G, int EPL>
 __global__ __launch_bounds__(256) void k_wave(StepArgs a, const int4* __restrict__ verts,
                                               int32_t nv) {
-    wave_body<H, W, G, EPL>(a, verts, nv, blockIdx.x);
+    __shared__ unsigned long long s_tab[4][kWaveCap];
+    wave_body<H, W, G, EPL>(a, verts, nv, wave_unit(blockIdx.x), s_tab[threadIdx.x >> 6]);
+}
+
+// The packed and half tiers of a step in one persistent launch (tuning
+// fused_small=1): each wave takes kSmallGrab consecutive units per atomic on
+// a counter until none are left, the heavier half-tier units first.  (One
+// atomic per unit serialises millions of them on one address.)  The two
+// tiers' per-wave LDS areas overlap in a union.  Two half units per wave
+// unit keep the kernel at 7 waves per SIMD; four would take 92 VGPRs (5
+// waves).  Measured slower than one launch per tier replayed from a HIP
+// graph (the batched planted mix: 7.8 against 6.8 ms per step), because the
+// separate half launch keeps four units in flight per wave at 6 waves per
+// SIMD; it is kept as an option the autotuner tries.
+union SmallLds {
+    PackedLds pk;
+    HalfLds<16> hf;
+};
+template <class H, bool W, bool G>
+__global__ __launch_bounds__(256) void k_small_tiers(StepArgs a, SmallTiers st) {
+    __shared__ SmallLds s_lds[4];
+    SmallLds& my = s_lds[threadIdx.x >> 6];
+    for (;;) {
+        uint32_t u0 = 0u;
+        if (lane_id() == 0) u0 = atomicAdd(st.counter, kSmallGrab);
+        u0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)u0);
+        if (u0 >= st.end[2]) break;  // wave-uniform
+        const uint32_t u1 = min(u0 + kSmallGrab, st.end[2]);
+        for (uint32_t u = u0; u < u1; ++u) {
+            if (u < st.end[0])
+                half_body<H, W, G, kSmallNP, 16, 4>(a, st.half_long, st.n_half_long, u, my.hf);
+            else if (u < st.end[1])
+                half_body<H, W, G, kSmallNP, 16, 2>(a, st.half_short, st.n_half_short, u - st.end[0], my.hf);
+            else
+                packed_body<H, W, G, kSmallNG>(a, st.groups, st.ngroups, st.nge, u - st.end[1], my.pk);
+            wave_lds_sync();  // the next unit's tier reuses the area
+        }
+    }
 }
 template <class H, bool W, bool G, int NT, int K = 8>
 __global__ __launch_bounds__(NT) void k_block(StepArgs a, const int32_t* __restrict__ verts,
@@ -1253,11 +1309,15 @@ __global__ __launch_bounds__(NT) void k_multi(const JobDesc* __restrict__ jobs,
     const uint32_t blk = blockIdx.x - (uint32_t)cum[j];
     const JobDesc& jd = jobs[j];
     if constexpr (L == kLaunchPacked) {
-        packed_body<H, W, G, kFoldNG>(jd.a, jd.groups, jd.ngroups, jd.ngroups_e, blk);
+        __shared__ PackedLds s_pk[4];
+        packed_body<H, W, G, kFoldNG>(jd.a, jd.groups, jd.ngroups, jd.ngroups_e, wave_unit(blk),
+                                      s_pk[threadIdx.x >> 6]);
     } else if constexpr (L == kLaunchHalf) {
-        half_body<H, W, G, kFoldNG>(jd.a, jd.half, jd.nhalf, blk);
+        __shared__ HalfLds<32> s_half[4];
+        half_body<H, W, G, kFoldNG>(jd.a, jd.half, jd.nhalf, wave_unit(blk), s_half[threadIdx.x >> 6]);
     } else if constexpr (L == kLaunchWave) {
-        wave_body<H, W, G>(jd.a, jd.wave, jd.nwave, blk);
+        __shared__ unsigned long long s_tab[4][kWaveCap];
+        wave_body<H, W, G>(jd.a, jd.wave, jd.nwave, wave_unit(blk), s_tab[threadIdx.x >> 6]);
     } else if constexpr (L >= kLaunchBlock0 && L < kLaunchBlock0 + kMaxBlockClasses) {
         block_body<H, W, G, NT>(jd.a, jd.block[L - kLaunchBlock0], cap_max, blk);
     } else if constexpr (L == kLaunchHubAgg) {
@@ -1803,6 +1863,17 @@ static void launch_half(const StepArgs& a, const int4* verts, int32_t nh, hipStr
     }
 }
 
+// Workgroups of `threads` threads of `fn` that the device holds at once.
+static int64_t resident_blocks(const void* fn, int threads) {
+    int per_cu = 0, cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0) != hipSuccess || per_cu < 1 ||
+        cus < 1)
+        return 1024;
+    return (int64_t)per_cu * cus;
+}
+
 template <class H, bool W, bool G>
 static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepArgs& a,
                         hipStream_t s, hipEvent_t* ev) {
@@ -1812,25 +1883,53 @@ static int launch_tiers(const DeviceGraph& g, const HubScratch& hs, const StepAr
     };
     mark(GLP_TIER_PACKED, 0);
     const int32_t ng = (int32_t)(H::kSkipIsolated ? p.groups_with_edges : (int64_t)p.groups.size());
-    if (ng > 0) {
-        const int32_t nge = (int32_t)p.groups_with_edges;
-        switch (units_per_wave(ng)) {
-            case 4:
-                hipLaunchKernelGGL((k_packed<H, W, G, 4>), dim3(grid_for(ng, 16)), dim3(256), 0, s, a,
-                                   g.d_groups, ng, nge);
-                break;
-            case 2:
-                hipLaunchKernelGGL((k_packed<H, W, G, 2>), dim3(grid_for(ng, 8)), dim3(256), 0, s, a,
-                                   g.d_groups, ng, nge);
-                break;
-            default:
-                hipLaunchKernelGGL((k_packed<H, W, G, 1>), dim3(grid_for(ng, 4)), dim3(256), 0, s, a,
-                                   g.d_groups, ng, nge);
+    if (p.tune.fused_small > 0 && hs.work) {
+        // packed and half tiers in one persistent launch
+        SmallTiers st;
+        const int32_t hs_n = (int32_t)p.half_short;
+        st.half_short = g.d_half;
+        st.n_half_short = hs_n;
+        st.half_long = g.d_half + hs_n;
+        st.n_half_long = (int32_t)p.half.size() - hs_n;
+        st.groups = g.d_groups;
+        st.ngroups = ng;
+        st.nge = (int32_t)p.groups_with_edges;
+        auto units = [](int64_t items, int per) { return (uint32_t)((items + per - 1) / per); };
+        st.end[0] = units(st.n_half_long, 4 * kSmallNP);
+        st.end[1] = st.end[0] + units(st.n_half_short, 4 * kSmallNP);
+        st.end[2] = st.end[1] + units(ng, kSmallNG);
+        st.counter = hs.work;
+        if (st.end[2] > 0) {
+            int rc = launch_zero_bytes(hs.work, 4, s);
+            if (rc != GLP_OK) return rc;
+            // twice what the chip holds at once: workgroups that find the
+            // units taken exit at once
+            static const int64_t resident = resident_blocks((const void*)k_small_tiers<H, W, G>, 256);
+            const unsigned grid =
+                (unsigned)std::min<int64_t>((st.end[2] + 4 * kSmallGrab - 1) / (4 * kSmallGrab), 2 * resident);
+            hipLaunchKernelGGL((k_small_tiers<H, W, G>), dim3(grid), dim3(256), 0, s, a, st);
+        }
+    } else {
+        if (ng > 0) {
+            const int32_t nge = (int32_t)p.groups_with_edges;
+            switch (units_per_wave(ng)) {
+                case 4:
+                    hipLaunchKernelGGL((k_packed<H, W, G, 4>), dim3(grid_for(ng, 16)), dim3(256), 0, s, a,
+                                       g.d_groups, ng, nge);
+                    break;
+                case 2:
+                    hipLaunchKernelGGL((k_packed<H, W, G, 2>), dim3(grid_for(ng, 8)), dim3(256), 0, s, a,
+                                       g.d_groups, ng, nge);
+                    break;
+                default:
+                    hipLaunchKernelGGL((k_packed<H, W, G, 1>), dim3(grid_for(ng, 4)), dim3(256), 0, s, a,
+                                       g.d_groups, ng, nge);
+            }
         }
     }
     mark(GLP_TIER_PACKED, 1);
     mark(GLP_TIER_WAVE, 0);
-    if (!p.half.empty()) {
+    if (!p.half.empty() && !(p.tune.fused_small > 0 && hs.work)) {
         const int32_t nh = (int32_t)p.half.size();
         // 16 lanes per vertex halve the instructions per vertex and double
         // the vertices in flight per wave; they pay off once the launch has

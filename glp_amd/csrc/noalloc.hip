@@ -78,6 +78,7 @@ size_t place(Arena& ar, const HostCSR& g, const Plan& p, bool layered, Layout* l
         lay->hs.lstart = ar.take<uint32_t>((size_t)p.hist_slots);
         lay->hs.best = ar.take<unsigned long long>(p.hubs.size());
     }
+    lay->hs.work = ar.take<uint32_t>(1);
     lay->changes = ar.take<unsigned long long>(kChangeSlots);
     return ar.at;
 }

@@ -30,7 +30,7 @@ const TuningField kFields[] = {
     {"wave_max_deg", &Tuning::wave_max_deg},
     {"block_max_deg", &Tuning::block_max_deg}, {"hub_chunk", &Tuning::hub_chunk},
     {"part_keys", &Tuning::part_keys},       {"part_limit", &Tuning::part_limit},
-    {"block_threads", &Tuning::block_threads},
+    {"block_threads", &Tuning::block_threads}, {"fused_small", &Tuning::fused_small},
 };
 }  // namespace
 
@@ -95,6 +95,7 @@ static int check_tuning(const Tuning& t) {
         return set_error(GLP_E_ARG, "part_limit must be in [0, part_keys]");
     if (t.block_threads != 256 && t.block_threads != 512 && t.block_threads != 1024)
         return set_error(GLP_E_ARG, "block_threads must be 256, 512 or 1024");
+    if (t.fused_small != 0 && t.fused_small != 1) return set_error(GLP_E_ARG, "fused_small must be 0 or 1");
     return GLP_OK;
 }
 

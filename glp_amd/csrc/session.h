@@ -23,7 +23,8 @@ struct glp_session {
     int32_t* adj = nullptr;              // pre-gathered neighbour labels (gather mode)
     int32_t* adj_alt = nullptr;          // the second adj of the two-step lookahead
     int32_t* mem = nullptr;              // SLPA memory (T+1) x n
-    void* cm_scratch = nullptr;          // counting-method scratch
+    int32_t* wide = nullptr;             // count-min / frequent / buffer: vertices above the narrow kernel's degree
+    int32_t nwide = 0;
     uint8_t* act[2] = {nullptr, nullptr};  // frontier flags, double-buffered like L
     int32_t* fctl = nullptr;             // frontier control {all_active, mark, threshold, gather words}
     int64_t act_bytes = 0;
@@ -64,6 +65,7 @@ int session_labels(glp_session* s, int32_t* out);
 int enqueue_step(glp_session* s, hipEvent_t* ev);
 // count-min / frequent / buffer steps (counting.hip)
 int enqueue_counting_step(glp_session* s);
+int counting_wide_list(glp_session* s);
 // SLPA post-processing (counting.hip)
 int slpa_post(glp_session* s, int32_t kmax, int32_t* dominant, int32_t* nsets, int32_t* sets);
 }  // namespace glp

@@ -220,10 +220,11 @@ glp_session::~glp_session() {
     (void)hipFree(seeds);
     (void)hipFree(fctl);
     (void)hipFree(hs.best);
+    (void)hipFree(hs.work);
     (void)hipFree(d_acc);
     (void)hipFree(d_ring);
     (void)hipFree(d_ctr);
-    (void)hipFree(cm_scratch);
+    (void)hipFree(wide);
     for (auto& set : ev_pool)
         for (auto& e : set)
             if (e) (void)hipEventDestroy(e);
@@ -246,7 +247,7 @@ size_t session_bytes(const HostCSR& g, const Plan& p, const glp_params& prm) {
     if (prm.frontier && prm.gather) b += 2 * ((n + 31) / 32) * 4 + push_entry_bytes(g.n, g.m);
     if (prm.frontier == 2 && prm.gather) b += (size_t)g.m * 4;  // the lookahead's second adj
     b += hub_scratch_bytes(p, prm.frontier != 0);
-    b += 8 * kChangeSlots + 8 * kChangeRing + 4;
+    b += 8 * kChangeSlots + 8 * kChangeRing + 4 + 4;  // change counters and ring, step and unit counters
     return b;
 }
 
@@ -320,12 +321,14 @@ int session_create(glp_graph* g, const glp_params* prm, glp_session** out) {
             }
         }
     }
+    GLP_HIP(hipMalloc(&s->hs.work, 4));  // the persistent small-tier launch's counter
     GLP_HIP(hipMalloc(&s->d_acc, 8 * kChangeSlots));
     GLP_HIP(hipMalloc(&s->d_ring, 8 * kChangeRing));
     GLP_HIP(hipMalloc(&s->d_ctr, 4));
     GLP_HIP(hipEventCreate(&s->ev_start));
     GLP_HIP(hipEventCreate(&s->ev_stop));
     s->bytes = session_bytes(g->csr, p, *prm);
+    if (counting_method(prm->method) && (rc = counting_wide_list(s.get())) != GLP_OK) return rc;
     for (int t = 0; t < 4; ++t) {
         s->timing.tier_vertices[t] = p.verts[t];
         s->timing.tier_edges[t] = p.edges[t];

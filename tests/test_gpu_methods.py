@@ -191,9 +191,11 @@ def test_countmin_hand_collisions(a, b, ca, rows, aliased, want):
 
 
 def test_countmin_largest_sketch():
-    """rows x width = 5 x 4096 (80 KB of LDS, the largest accepted shape)."""
-    g = G.Graph.rmat(11, 16, seed=34)
+    """rows x width = 5 x 4096 (80 KB of LDS, the largest accepted shape), in
+    the one-wavefront and the 1024-thread (degree > 1024) launches."""
+    g = G.Graph.rmat(13, 16, seed=34)
     off, nbr, _ = g.csr()
+    assert np.diff(off).max() > 1024
     labels, _ = G.run(g, 2, method=G.COUNTMIN, cm_rows=5, cm_width=4096)
     prev = np.arange(len(off) - 1, dtype=np.int32)
     for _ in range(2):
@@ -301,6 +303,29 @@ def test_seeded_session_replay_checkpoint_and_errors(rmat14):
         G.Batch(mix_graphs(2, 1), method=G.SEEDED)
 
 
+# --------------------------------------------------- persistent small tiers ----
+
+@pytest.mark.parametrize("method,kw", [
+    (G.CLASSIC, {}), (G.CLASSIC, {"weighted": True}), (G.CLASSIC, {"gather": True}),
+    (G.LAYERED, {"gamma": (0, 1)}), (G.LAYERED, {"gamma": (1, 4), "weighted": True}),
+    (G.SLPA, {"seed": 11}), (G.USER, {}), (G.SEEDED, {"seeds": "5%"})])
+def test_fused_small_tiers_equal_separate_launches(method, kw):
+    """fused_small=1 (packed, half and wave tiers in one persistent launch
+    whose waves take units from a counter) gives every step's labels of one
+    launch per tier (fused_small=0), with every tier populated."""
+    kw = dict(kw)
+    dumps = []
+    for fused in (0, 1):
+        with G.tuning(fused_small=fused):
+            g = G.Graph.rmat(13, 16, seed=32, wmax=8)   # the tuning is read at the graph's upload
+            if isinstance(kw.get("seeds"), str):
+                kw["seeds"] = (np.random.default_rng(43).random(g.n) < 0.05).astype(np.uint8)
+            _, dump, t = G.run(g, 5, dump=True, method=method, **kw)
+            assert all(t["tier_vertices"][k] > 0 for k in ("packed", "wave", "block"))
+            dumps.append(dump)
+    assert_steps_equal(dumps[1], dumps[0], "fused small tiers")
+
+
 # ------------------------------------------------------------------ batched ----
 
 def mix_graphs(count, seed, nmax=6000):
@@ -337,15 +362,17 @@ HUB_TUNING = dict(pack_max_deg=8, wave_max_deg=16, block_max_deg=32, hub_chunk=2
 
 @pytest.mark.parametrize("method,kw", [(G.CLASSIC, {}), (G.LAYERED, {"gamma": (1, 4)}),
                                        (G.FREQUENT, {"table_cap": 8})])
-@pytest.mark.parametrize("tuned", [False, True])
+@pytest.mark.parametrize("tuned", [False, True, "fused"])
 def test_batch_equals_alone_and_oracle(method, kw, tuned):
     """Every graph of a batch (union stepped in one launch per tier, its HIP
     graph replay, and the plain per-graph form) matches the oracle and the
-    graph run alone."""
-    with G.tuning(**(HUB_TUNING if tuned else {})):
+    graph run alone.  "fused": the packed and half tiers in one persistent
+    launch."""
+    tuning = {False: {}, True: HUB_TUNING, "fused": {"fused_small": 1}}[tuned]
+    with G.tuning(**tuning):
         gs = mix_graphs(24, 5)
         gs.append(G.Graph.rmat(12, 16, seed=2))     # max degree 1339: hub tier when tuned
-        if tuned:
+        if tuned is True:
             _, t = G.run(gs[-1], 1, method=method, **kw)
             assert method == G.FREQUENT or t["tier_vertices"]["hub"] > 0
         want = [oracle_labels(g, 6, method, kw) for g in gs]
