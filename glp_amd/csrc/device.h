@@ -217,19 +217,25 @@ inline int launch_zero_bytes(void* p, int64_t count, hipStream_t s) {
     return launch_fill_i32(static_cast<int32_t*>(p), 0, count / 4, s);
 }
 
-// One layered job's written labels and write-side volumes (folded volume
-// launches of the serving path).
+// One layered job's labels and volumes for the folded volume launches of
+// the serving path: the written labels and write-side volumes (zero, fill),
+// or the read labels, read-side volumes and the records to write (records).
 struct VolumeJob {
     const int32_t* L;
     int32_t* V;
     int32_t n;
     int32_t pad;
+    int2* LV;
 };
-// Workgroups a job of n vertices needs in the folded zero and volume launches.
+// Workgroups a job of n vertices needs in the folded zero (and records) and
+// volume launches.
 void volume_job_blocks(int32_t n, int32_t* zero_blocks, int32_t* volume_blocks);
-// zero: V = 0 for every job (before the tiers); else V += histogram of L
-// (after them).  cum: nj + 1 prefix sums of the job's workgroups; grid = cum[nj].
-int launch_multi_volume(bool zero, const VolumeJob* jobs, const int32_t* cum, int32_t nj, int32_t grid,
+// kVolumeZero: V = 0 for every job (before the tiers); kVolumeRecords:
+// LV[u] = {L[u], V[L[u]]} (before the tiers); kVolumeFill: V += histogram
+// of L (after them).  cum: nj + 1 prefix sums of the job's workgroups;
+// grid = cum[nj].
+enum VolumeMode { kVolumeZero, kVolumeRecords, kVolumeFill };
+int launch_multi_volume(VolumeMode mode, const VolumeJob* jobs, const int32_t* cum, int32_t nj, int32_t grid,
                         hipStream_t s);
 
 // Device CSR built on the device (gen_device.hip); n + 1 offsets, m
@@ -249,9 +255,10 @@ struct DeviceCSR {
 int gen_rmat_device(int32_t scale, int32_t edge_factor, double a, double b, double c, uint64_t seed,
                     int32_t wmax, hipStream_t s, DeviceCSR* out);
 
-// Folded step for nj jobs with layered/classic hooks (gamma 0 = classic).
+// Folded step for nj jobs with the layered hooks (gamma 0 = classic) or,
+// slot_scores, the slot-score hooks (layered, gamma > 0: reads a.LV).
 // jobs and cum are device arrays; grid (host) holds each launch's size.
-int launch_multi(bool weighted, const JobDesc* jobs, const int32_t* cum, int32_t nj,
+int launch_multi(bool weighted, bool slot_scores, const JobDesc* jobs, const int32_t* cum, int32_t nj,
                  const int32_t* grid, const Tuning& t, const int32_t* class_cap, hipStream_t s);
 // Workgroups each folded launch needs for a job with plan p.
 void job_blocks(const Plan& p, int32_t* blocks);

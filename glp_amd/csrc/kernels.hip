@@ -1381,8 +1381,12 @@ static int launch_multi_t(const JobDesc* jobs, const int32_t* cum, int32_t nj, c
     return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "folded launch: %s", hipGetErrorString(e));
 }
 
-int launch_multi(bool weighted, const JobDesc* jobs, const int32_t* cum, int32_t nj,
+int launch_multi(bool weighted, bool slot_scores, const JobDesc* jobs, const int32_t* cum, int32_t nj,
                  const int32_t* grid, const Tuning& t, const int32_t* class_cap, hipStream_t s) {
+    if (slot_scores) {
+        if (weighted) return launch_multi_t<LayeredVolHooks, true>(jobs, cum, nj, grid, t, class_cap, s);
+        return launch_multi_t<LayeredVolHooks, false>(jobs, cum, nj, grid, t, class_cap, s);
+    }
     if (weighted) return launch_multi_t<LayeredHooks, true>(jobs, cum, nj, grid, t, class_cap, s);
     return launch_multi_t<LayeredHooks, false>(jobs, cum, nj, grid, t, class_cap, s);
 }
@@ -1390,8 +1394,9 @@ int launch_multi(bool weighted, const JobDesc* jobs, const int32_t* cum, int32_t
 // Workgroups each launch of a folded step needs for one job.
 void job_blocks(const Plan& p, int32_t* blocks) {
     for (int L = 0; L < kLaunches; ++L) blocks[L] = 0;
-    // folded jobs run the layered hooks, which skip edge-less groups
-    static_assert(LayeredHooks::kSkipIsolated, "folded launches run groups with edges only");
+    // folded jobs run the layered or slot-score hooks, which skip edge-less groups
+    static_assert(LayeredHooks::kSkipIsolated && LayeredVolHooks::kSkipIsolated,
+                  "folded launches run groups with edges only");
     blocks[kLaunchPacked] = (int32_t)((p.groups_with_edges + 4 * kFoldNG - 1) / (4 * kFoldNG));
     blocks[kLaunchHalf] = (int32_t)((p.half.size() + 8 * kFoldNG - 1) / (8 * kFoldNG));
     blocks[kLaunchWave] = (int32_t)((p.wave.size() + 4 * kWaveNV - 1) / (4 * kWaveNV));
@@ -1758,16 +1763,30 @@ __global__ __launch_bounds__(kVolThreads) void k_multi_volume(const VolumeJob* _
     const int32_t j = job_of(cum, nj, blockIdx.x);
     volume_body(jobs[j].L, jobs[j].V, jobs[j].n, blockIdx.x - (uint32_t)cum[j]);
 }
+// The slot-score jobs' records LV[u] = {L[u], V[L[u]]}, kZeroPerBlock rows
+// per workgroup.
+__global__ __launch_bounds__(256) void k_multi_records(const VolumeJob* __restrict__ jobs,
+                                                       const int32_t* __restrict__ cum, int32_t nj) {
+    const int32_t j = job_of(cum, nj, blockIdx.x);
+    const VolumeJob& jb = jobs[j];
+    const int64_t b0 = (int64_t)(blockIdx.x - (uint32_t)cum[j]) * kZeroPerBlock;
+    const int64_t b1 = min((int64_t)jb.n, b0 + kZeroPerBlock);
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += 256) {
+        const int32_t l = jb.L[i];
+        jb.LV[i] = make_int2(l, jb.V[l]);
+    }
+}
 
 void volume_job_blocks(int32_t n, int32_t* zero_blocks, int32_t* volume_blocks) {
     *zero_blocks = (int32_t)(((int64_t)n + kZeroPerBlock - 1) / kZeroPerBlock);
     *volume_blocks = (int32_t)(((int64_t)n + kVolChunk - 1) / kVolChunk);
 }
 
-int launch_multi_volume(bool zero, const VolumeJob* jobs, const int32_t* cum, int32_t nj, int32_t grid,
+int launch_multi_volume(VolumeMode mode, const VolumeJob* jobs, const int32_t* cum, int32_t nj, int32_t grid,
                         hipStream_t s) {
     if (nj == 0 || grid == 0) return GLP_OK;
-    if (zero) hipLaunchKernelGGL(k_multi_zero, dim3(grid), dim3(256), 0, s, jobs, cum, nj);
+    if (mode == kVolumeZero) hipLaunchKernelGGL(k_multi_zero, dim3(grid), dim3(256), 0, s, jobs, cum, nj);
+    else if (mode == kVolumeRecords) hipLaunchKernelGGL(k_multi_records, dim3(grid), dim3(256), 0, s, jobs, cum, nj);
     else hipLaunchKernelGGL(k_multi_volume, dim3(grid), dim3(kVolThreads), 0, s, jobs, cum, nj);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? GLP_OK : set_error(GLP_E_HIP, "folded volume launch: %s", hipGetErrorString(e));
@@ -2086,6 +2105,8 @@ void init_kernels() {
 #undef GLP_SETLDS
 #define GLP_SETLDS_MULTI(W, L, NT)                                                              \
     (void)hipFuncSetAttribute((const void*)k_multi<LayeredHooks, W, false, L, NT>,              \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim);                 \
+    (void)hipFuncSetAttribute((const void*)k_multi<LayeredVolHooks, W, false, L, NT>,           \
                               hipFuncAttributeMaxDynamicSharedMemorySize, lim);
 #define GLP_SETLDS_MULTI_C(W, C) GLP_SETLDS_MULTI(W, kLaunchBlock0 + C, 256)                    \
     GLP_SETLDS_MULTI(W, kLaunchBlock0 + C, 512) GLP_SETLDS_MULTI(W, kLaunchBlock0 + C, 1024)

@@ -11,7 +11,10 @@
 //  * the worker owns the tick stream.  Every tick advances every running job
 //    by one step with one launch per tier, the jobs folded together through
 //    a device-side job table (k_multi), plus one folded launch that zeroes
-//    and one that fills the layered jobs' volumes.  A tick's table goes up
+//    and one that fills the layered jobs' volumes.  Layered jobs with
+//    gamma > 0 run the slot-score hooks from a second table (one more set of
+//    tier launches), after a folded launch writes their {label, volume}
+//    records.  A tick's table goes up
 //    in one copy from a ring of three pinned staging slots; the worker waits
 //    only for the tick that last used a slot, so two ticks stay queued on
 //    the device while it prepares the next.  Jobs admitted meanwhile start
@@ -24,7 +27,7 @@
 // admission and completion make no hipMalloc / hipFree once the pool has
 // warmed up (hipFree waits for the whole device).  Classic jobs run the
 // layered hooks with gamma = 0, which is classic LP, so one kernel set
-// serves both kinds.  Results do not depend on what else runs: each job's
+// serves both kinds (and gamma = 0 layered jobs).  Results do not depend on what else runs: each job's
 // arithmetic touches only its own arrays.  Every field the threads share is
 // read and written under `mu` or is atomic.
 
@@ -126,6 +129,7 @@ struct ServerJob {
     size_t block_cap = 0;
     int32_t* L[2] = {nullptr, nullptr};
     int32_t* V[2] = {nullptr, nullptr};
+    int2* LV = nullptr;  // layered, gamma > 0: per-step {label, volume of the label} records
     HubScratch hs;
     unsigned long long* changes = nullptr;
     int cur = 0;
@@ -135,10 +139,13 @@ struct ServerJob {
 
 // Sub-allocation of one job's state in one pooled block.
 struct JobLayout {
-    size_t L[2], V[2], pairs, hist, lstart, best, changes, bytes;
+    size_t L[2], V[2], LV, pairs, hist, lstart, best, changes, bytes;
 };
 
-static JobLayout job_layout(int32_t n, const Plan& p, bool layered) {
+// Slot-score (gamma > 0) layered jobs keep per-vertex {label, volume} records.
+static bool job_records(const glp_params& p) { return p.method == GLP_LAYERED && p.gamma_num > 0; }
+
+static JobLayout job_layout(int32_t n, const Plan& p, bool layered, bool records) {
     JobLayout o{};
     size_t at = 0;
     auto take = [&](size_t b) {
@@ -151,6 +158,7 @@ static JobLayout job_layout(int32_t n, const Plan& p, bool layered) {
     o.L[1] = take(nn * 4);
     o.V[0] = layered ? take(nn * 4) : 0;
     o.V[1] = layered ? take(nn * 4) : 0;
+    o.LV = records ? take(nn * 8) : 0;
     o.pairs = take((size_t)p.pair_slots * 8);
     o.hist = take((size_t)p.hist_slots * 4);
     o.lstart = take((size_t)p.hist_slots * 4);
@@ -265,7 +273,8 @@ int glp_server::start_job(ServerJob* j) {
     const DeviceGraph& dg = *j->rg->dg;
     const Plan& p = dg.plan;
     const bool layered = j->p.method == GLP_LAYERED;
-    const JobLayout lay = job_layout(dg.n, p, layered);
+    const bool records = job_records(j->p);
+    const JobLayout lay = job_layout(dg.n, p, layered, records);
     int rc = pool.alloc(lay.bytes, &j->block, &j->block_cap);
     if (rc != GLP_OK) return rc;
     char* base = static_cast<char*>(j->block);
@@ -273,6 +282,7 @@ int glp_server::start_job(ServerJob* j) {
     j->L[1] = reinterpret_cast<int32_t*>(base + lay.L[1]);
     j->V[0] = layered ? reinterpret_cast<int32_t*>(base + lay.V[0]) : nullptr;
     j->V[1] = layered ? reinterpret_cast<int32_t*>(base + lay.V[1]) : nullptr;
+    j->LV = records ? reinterpret_cast<int2*>(base + lay.LV) : nullptr;
     if (!p.hubs.empty()) {
         j->hs.pairs = reinterpret_cast<unsigned long long*>(base + lay.pairs);
         j->hs.hist = reinterpret_cast<uint32_t*>(base + lay.hist);
@@ -303,22 +313,37 @@ void glp_server::hand_over(ServerJob* j) {
 }
 
 // Enqueues one step of every running job with folded launches.
+//
+// Two job tables per tick: classic and gamma = 0 layered jobs run the
+// layered hooks (gamma = 0 reads no volume), layered jobs with gamma > 0 the
+// slot-score hooks, which read each neighbour's {label, volume} record
+// (written for those jobs by one folded launch before the tiers) and score
+// in the table slots.
 int glp_server::tick() {
     std::vector<ServerJob*> act;
     for (auto* j : running)
         if (j->done_steps < j->steps && !j->cancel) act.push_back(j);
+    if (act.empty()) return GLP_OK;
+    // slot-score jobs last: jobs [0, na) form table A, [na, nj) table B
+    std::stable_partition(act.begin(), act.end(), [](const ServerJob* j) { return j->LV == nullptr; });
     const int32_t nj = (int32_t)act.size();
-    if (nj == 0) return GLP_OK;
-    int32_t nl = 0;
-    for (auto* j : act) nl += j->p.method == GLP_LAYERED ? 1 : 0;
-    // packed layout of this tick's table
+    int32_t na = 0, nl = 0;
+    for (auto* j : act) {
+        na += j->LV == nullptr ? 1 : 0;
+        nl += j->p.method == GLP_LAYERED ? 1 : 0;
+    }
+    const int32_t nb = nj - na;
+    // packed layout of this tick's tables
     auto up16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
     const size_t o_jobs = 0;
-    const size_t o_cum = up16(o_jobs + sizeof(JobDesc) * nj);
-    const size_t o_vol = up16(o_cum + sizeof(int32_t) * kLaunches * (nj + 1));
+    const size_t o_cum_a = up16(o_jobs + sizeof(JobDesc) * nj);
+    const size_t o_cum_b = up16(o_cum_a + sizeof(int32_t) * kLaunches * (na + 1));
+    const size_t o_vol = up16(o_cum_b + sizeof(int32_t) * kLaunches * (nb + 1));
     const size_t o_zc = up16(o_vol + sizeof(VolumeJob) * std::max(nl, 1));
     const size_t o_vc = up16(o_zc + sizeof(int32_t) * (nl + 1));
-    const size_t total = up16(o_vc + sizeof(int32_t) * (nl + 1));
+    const size_t o_rec = up16(o_vc + sizeof(int32_t) * (nl + 1));
+    const size_t o_rc = up16(o_rec + sizeof(VolumeJob) * std::max(nb, 1));
+    const size_t total = up16(o_rc + sizeof(int32_t) * (nb + 1));
     TickSlot& sl = slots[next_slot];
     next_slot = (next_slot + 1) % kTickSlots;
     if (sl.used) GLP_HIP(hipEventSynchronize(sl.ev));  // the tick that last used this slot is done
@@ -334,14 +359,16 @@ int glp_server::tick() {
         sl.cap = cap;
     }
     JobDesc* h_jobs = reinterpret_cast<JobDesc*>(sl.host + o_jobs);
-    int32_t* h_cum = reinterpret_cast<int32_t*>(sl.host + o_cum);
     VolumeJob* h_vol = reinterpret_cast<VolumeJob*>(sl.host + o_vol);
     int32_t* h_zc = reinterpret_cast<int32_t*>(sl.host + o_zc);
     int32_t* h_vc = reinterpret_cast<int32_t*>(sl.host + o_vc);
-    bool weighted = false;
-    int32_t li = 0;
+    VolumeJob* h_rec = reinterpret_cast<VolumeJob*>(sl.host + o_rec);
+    int32_t* h_rc = reinterpret_cast<int32_t*>(sl.host + o_rc);
+    bool weighted_a = false, weighted_b = false;
+    int32_t li = 0, ri = 0;
     h_zc[0] = 0;
     h_vc[0] = 0;
+    h_rc[0] = 0;
     for (int32_t k = 0; k < nj; ++k) {
         ServerJob* j = act[k];
         const DeviceGraph& dg = *j->rg->dg;
@@ -363,19 +390,25 @@ int glp_server::tick() {
             a.Vw = j->V[j->cur ^ 1];
             a.score_mul = j->p.gamma_den + j->p.gamma_num;
             a.score_vmul = j->p.gamma_num;
-            h_vol[li] = VolumeJob{a.Lw, a.Vw, dg.n, 0};
+            h_vol[li] = VolumeJob{a.Lw, a.Vw, dg.n, 0, nullptr};
             int32_t zb = 0, vb = 0;
             volume_job_blocks(dg.n, &zb, &vb);
             h_zc[li + 1] = h_zc[li] + zb;
             h_vc[li + 1] = h_vc[li] + vb;
             ++li;
+            if (j->LV) {
+                a.LV = j->LV;
+                h_rec[ri] = VolumeJob{a.L, j->V[j->cur], dg.n, 0, j->LV};  // read-side volumes
+                h_rc[ri + 1] = h_rc[ri] + zb;  // the records take the zero launch's rows per workgroup
+                ++ri;
+            }
         } else {
             a.score_mul = 1;
             a.score_vmul = 0;
         }
-        weighted |= j->p.weighted != 0;
+        (k < na ? weighted_a : weighted_b) |= j->p.weighted != 0;
         d.groups = dg.d_groups;
-        d.ngroups = (int32_t)p.groups_with_edges;  // layered hooks skip edge-less groups
+        d.ngroups = (int32_t)p.groups_with_edges;  // both hook sets skip edge-less groups
         d.ngroups_e = (int32_t)p.groups_with_edges;
         d.half = dg.d_half;
         d.nhalf = (int32_t)p.half.size();
@@ -397,24 +430,35 @@ int glp_server::tick() {
         h.hs = j->hs;
         job_blocks(p, d.blocks);
     }
-    int32_t grid[kLaunches];
-    for (int L = 0; L < kLaunches; ++L) {
-        int32_t* row = h_cum + (size_t)L * (nj + 1);
-        row[0] = 0;
-        for (int32_t k = 0; k < nj; ++k) row[k + 1] = row[k] + h_jobs[k].blocks[L];
-        grid[L] = row[nj];
-    }
+    // per launch, the prefix sums of each table's workgroups
+    int32_t grid_a[kLaunches], grid_b[kLaunches];
+    auto prefix = [&](size_t off, int32_t first, int32_t cnt, int32_t* grid) {
+        int32_t* rows = reinterpret_cast<int32_t*>(sl.host + off);
+        for (int L = 0; L < kLaunches; ++L) {
+            int32_t* row = rows + (size_t)L * (cnt + 1);
+            row[0] = 0;
+            for (int32_t k = 0; k < cnt; ++k) row[k + 1] = row[k] + h_jobs[first + k].blocks[L];
+            grid[L] = row[cnt];
+        }
+    };
+    prefix(o_cum_a, 0, na, grid_a);
+    prefix(o_cum_b, na, nb, grid_b);
     GLP_HIP(hipMemcpyAsync(sl.dev, sl.host, total, hipMemcpyHostToDevice, stream));
     const JobDesc* d_jobs = reinterpret_cast<const JobDesc*>(sl.dev + o_jobs);
-    const int32_t* d_cum = reinterpret_cast<const int32_t*>(sl.dev + o_cum);
     const VolumeJob* d_vol = reinterpret_cast<const VolumeJob*>(sl.dev + o_vol);
-    int rc = launch_multi_volume(true, d_vol, reinterpret_cast<const int32_t*>(sl.dev + o_zc), nl, h_zc[nl],
-                                 stream);
+    auto dev_i32 = [&](size_t off) { return reinterpret_cast<const int32_t*>(sl.dev + off); };
+    int rc = launch_multi_volume(kVolumeZero, d_vol, dev_i32(o_zc), nl, h_zc[nl], stream);
     if (rc != GLP_OK) return rc;
-    if ((rc = launch_multi(weighted, d_jobs, d_cum, nj, grid, tune, class_cap, stream)) != GLP_OK) return rc;
-    if ((rc = launch_multi_volume(false, d_vol, reinterpret_cast<const int32_t*>(sl.dev + o_vc), nl, h_vc[nl],
-                                  stream)) != GLP_OK)
+    if ((rc = launch_multi_volume(kVolumeRecords, reinterpret_cast<const VolumeJob*>(sl.dev + o_rec),
+                                  dev_i32(o_rc), nb, h_rc[nb], stream)) != GLP_OK)
         return rc;
+    if (na > 0 && (rc = launch_multi(weighted_a, false, d_jobs, dev_i32(o_cum_a), na, grid_a, tune, class_cap,
+                                     stream)) != GLP_OK)
+        return rc;
+    if (nb > 0 && (rc = launch_multi(weighted_b, true, d_jobs + na, dev_i32(o_cum_b), nb, grid_b, tune,
+                                     class_cap, stream)) != GLP_OK)
+        return rc;
+    if ((rc = launch_multi_volume(kVolumeFill, d_vol, dev_i32(o_vc), nl, h_vc[nl], stream)) != GLP_OK) return rc;
     GLP_HIP(hipEventRecord(sl.ev, stream));
     sl.used = true;
     for (auto* j : act) {
@@ -598,7 +642,7 @@ void glp_server::complete_loop() {
 
 // Bytes of a job's state (labels, volumes, hub scratch).
 static size_t job_state_bytes(const ResidentGraph& rg, const Plan& plan, const glp_params& p) {
-    return job_layout(rg.n, plan, p.method == GLP_LAYERED).bytes;
+    return job_layout(rg.n, plan, p.method == GLP_LAYERED, job_records(p)).bytes;
 }
 
 static int check_job_params(const ResidentGraph& rg, const glp_params* p) {

@@ -398,18 +398,19 @@ def test_batch_rejects_stream_methods():
 
 @pytest.mark.parametrize("tuned", [False, True])
 def test_server_results_equal_alone_in_the_mix(tuned):
-    """Weighted classic and layered (gamma 1/4) jobs served together under a
-    small budget: each result matches the oracle and the job run alone.
-    Tuned: the folded hub launches run (every graph has degrees > 32)."""
+    """Weighted classic and layered (gamma 1/4 and 0) jobs served together
+    under a small budget: each result matches the oracle and the job run
+    alone.  Tuned: the folded hub launches run (every graph has degrees >
+    32)."""
     rng = np.random.default_rng(3)
     graphs, params, steps = [], [], []
     with G.tuning(**(HUB_TUNING if tuned else {})):
         for i in range(12):
             scale = int(rng.integers(8, 13))
             g = G.Graph.rmat(scale, int(rng.integers(8, 17)), seed=100 + i, wmax=8)
-            layered = i % 2 == 1
+            layered = i % 2 == 1   # gamma 1/4 (slot-score table), or 0 (layered-hook table)
             p = G.make_params(method=G.LAYERED if layered else G.CLASSIC,
-                              gamma=(1, 4) if layered else (0, 1), weighted=True)
+                              gamma=(1, 4) if i % 4 == 1 else (0, 1), weighted=True)
             graphs.append(g)
             params.append(p)
             steps.append(int(rng.integers(3, 9)))
