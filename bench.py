@@ -458,17 +458,21 @@ def bench_batched(args, G):
                                            "block_max_deg", "block_threads", "hub_chunk", "part_keys",
                                            "fused_small")}
     log(("autotuned: %s" % chosen) if tuned_ms else "not autotuned")
-    # plain: one session per graph, stepped one graph after another
-    b.run(1, mode=2)
-    b.reset()
-    plain_ms = b.run(steps, mode=2)
-    plain_labels = [b.labels(i) for i in range(len(graphs))]
-    log("plain %.1f ms" % plain_ms)
+    plain_ms, plain_labels = None, None
+    if not args.batched_only:
+        # plain: one session per graph, stepped one graph after another
+        b.run(1, mode=2)
+        b.reset()
+        plain_ms = b.run(steps, mode=2)
+        plain_labels = [b.labels(i) for i in range(len(graphs))]
+        log("plain %.1f ms" % plain_ms)
     # batched: the packed batch, steps replayed from a captured HIP graph
     b.run(1, mode=1)
     b.reset()
     batch_ms = b.run(steps, mode=1)
-    same = all(np.array_equal(b.labels(i), plain_labels[i]) for i in range(len(graphs)))
+    same = None
+    if plain_labels is not None:
+        same = all(np.array_equal(b.labels(i), plain_labels[i]) for i in range(len(graphs)))
     log("batched %.1f ms, labels identical: %s" % (batch_ms, same))
     b.reset()
     eager_ms = b.run(steps, mode=0)
@@ -476,7 +480,7 @@ def bench_batched(args, G):
     for k, v in saved.items():
         G.tuning_set(k, v)
     gps = len(graphs) / (batch_ms / 1e3)
-    plain_gps = len(graphs) / (plain_ms / 1e3)
+    plain_gps = len(graphs) / (plain_ms / 1e3) if plain_ms else None
     return {
         "metric": "batched graphs/s vs plain (1,024-graph planted-partition mix, classic LP, %d steps)" % steps,
         "value": round(gps, 2), "unit": "graphs/s", "steps": steps,
@@ -486,9 +490,10 @@ def bench_batched(args, G):
                                "inter fraction U[.1,.4]), seed %d" % (len(graphs), args.seed),
                    "graphs": len(graphs), "n_total": n_tot, "m_total": m_tot,
                    "autotuned": tuned_ms is not None, "tuning": chosen},
-        "plain_graphs_per_s": round(plain_gps, 2), "speedup_vs_plain": round(gps / plain_gps, 3),
-        "batched_eager_ms": round(eager_ms, 2), "plain_ms": round(plain_ms, 2),
-        "batched_ms": round(batch_ms, 2), "labels_identical": bool(same),
+        "plain_graphs_per_s": round(plain_gps, 2) if plain_gps else None,
+        "speedup_vs_plain": round(gps / plain_gps, 3) if plain_gps else None,
+        "batched_eager_ms": round(eager_ms, 2), "plain_ms": round(plain_ms, 2) if plain_ms else None,
+        "batched_ms": round(batch_ms, 2), "labels_identical": same,
         "gteps_batched": round(m_tot * steps / (batch_ms / 1e3) / 1e9, 3),
     }
 
@@ -679,6 +684,8 @@ def main():
     ap.add_argument("--graphs", type=int, default=1024)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="batched workload without the plain per-graph comparison (for profiling)")
     ap.add_argument("--jobs", type=int, default=20000)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--slots", type=int, default=64)

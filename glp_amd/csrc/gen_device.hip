@@ -8,8 +8,9 @@
 // duplicates dropped, and the sorted unique keys are the CSR: neighbours
 // ascending per vertex, exactly the host's sort-and-unique per vertex.
 // Weights are the host's hash of the unordered pair, so w(u,v) = w(v,u).
-// The sort and the scans are rocPRIM device primitives (bench and serving
-// infrastructure; the LP step itself uses none).
+// The offsets come from the positions where the sorted keys' source
+// changes.  The sort and the unique are rocPRIM device primitives (bench and
+// serving infrastructure; the LP step itself uses none).
 
 #include "device.h"
 
@@ -17,7 +18,6 @@
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_select.hpp>
 
 namespace glp {
@@ -77,10 +77,18 @@ __global__ __launch_bounds__(256) void k_rmat_draw(RmatArgs r, unsigned long lon
 
 // Degree of each source over the m unique keys (the sentinel sorts last and
 // is excluded from m).
-__global__ __launch_bounds__(256) void k_key_degree(const unsigned long long* __restrict__ keys, int64_t m,
-                                                    int32_t scale, int32_t* __restrict__ deg) {
+// Offsets straight from the sorted keys: off[v] is the first position whose
+// source is >= v, so position i (0 <= i <= m) writes off[v] for the
+// sources src(i-1) < v <= src(i), with src(-1) = -1 and src(m) = n.  Every
+// entry is written once, and a hub costs nothing beyond its own positions
+// (a degree histogram's atomics serialise on a hub's counter).
+__global__ __launch_bounds__(256) void k_key_offsets(const unsigned long long* __restrict__ keys, int64_t m,
+                                                     int32_t scale, int32_t n, int32_t* __restrict__ off) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) atomicAdd(&deg[keys[i] >> scale], 1);
+    if (i > m) return;
+    const int64_t hi = i < m ? (int64_t)(keys[i] >> scale) : (int64_t)n;
+    const int64_t lo = i > 0 ? (int64_t)(keys[i - 1] >> scale) : -1;
+    for (int64_t v = lo + 1; v <= hi; ++v) off[v] = (int32_t)i;
 }
 
 // Neighbour ids, and weights 1 + H(min, max) % wmax (gen.cpp).
@@ -173,15 +181,6 @@ int gen_rmat_device(int32_t scale, int32_t edge_factor, double a, double b, doub
     }
     const int64_t m = (int64_t)nuniq - (nuniq > 0 && last == (1ull << (2 * scale)) ? 1 : 0);
     if (m > INT32_MAX) return set_error(GLP_E_RANGE, "edge count exceeds int32 CSR offsets");
-    // offsets: degree histogram, then an exclusive scan over n + 1 entries
-    int32_t* deg = nullptr;
-    GLP_HIP(hipMalloc(&sc.p[3], ((size_t)n + 2) * 4));
-    deg = static_cast<int32_t*>(sc.p[3]);
-    GLP_HIP(hipMemsetAsync(deg, 0, ((size_t)n + 2) * 4, s));
-    if (m > 0) {
-        hipLaunchKernelGGL(k_key_degree, dim3(grid_of(m)), dim3(256), 0, s, keys, m, scale, deg);
-        GLP_HIP(hipGetLastError());
-    }
     DeviceCSR o;
     o.n = n;
     o.m = m;
@@ -197,18 +196,8 @@ int gen_rmat_device(int32_t scale, int32_t edge_factor, double a, double b, doub
             }
         }
     } rel{&guard};
-    size_t scan_bytes = 0;
-    GLP_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, deg, o.off, 0, (size_t)n + 1,
-                                    rocprim::plus<int32_t>(), s));
-    void* scan_tmp = sc.p[2];  // the sort's temporary storage is free again
-    if (scan_bytes > tmp_bytes) {
-        GLP_HIP(hipFree(sc.p[2]));
-        sc.p[2] = nullptr;
-        GLP_HIP(hipMalloc(&sc.p[2], scan_bytes));
-        scan_tmp = sc.p[2];
-    }
-    GLP_HIP(rocprim::exclusive_scan(scan_tmp, scan_bytes, deg, o.off, 0, (size_t)n + 1,
-                                    rocprim::plus<int32_t>(), s));
+    hipLaunchKernelGGL(k_key_offsets, dim3(grid_of(m + 1)), dim3(256), 0, s, keys, m, scale, n, o.off);
+    GLP_HIP(hipGetLastError());
     GLP_HIP(hipMalloc(&o.nbr, std::max<size_t>((size_t)m, 1) * 4));
     if (wmax > 0) GLP_HIP(hipMalloc(&o.w, std::max<size_t>((size_t)m, 1) * 4));
     if (m > 0) {
@@ -216,7 +205,7 @@ int gen_rmat_device(int32_t scale, int32_t edge_factor, double a, double b, doub
                            seed ^ 0x5745494748545321ULL, (uint32_t)std::max(wmax, 1), o.nbr, o.w);
         GLP_HIP(hipGetLastError());
     }
-    int32_t* d_max = deg;  // the histogram is consumed: reuse its first word
+    int32_t* d_max = reinterpret_cast<int32_t*>(d_count);  // the unique count is read: reuse its word
     GLP_HIP(hipMemsetAsync(d_max, 0, 4, s));
     hipLaunchKernelGGL(k_max_degree, dim3(std::min<unsigned>(grid_of(n), 1024)), dim3(256), 0, s, o.off, n,
                        d_max);

@@ -25,7 +25,7 @@ from collections import defaultdict
 
 def family(kernel_name):
     """Short kernel name: 'void glp::k_block<...>(...)' -> 'k_block'."""
-    s = kernel_name.strip()
+    s = kernel_name.strip().replace("(anonymous namespace)::", "")
     if s.startswith("void "):
         s = s[5:]
     s = s.split("(")[0].split("<")[0]
