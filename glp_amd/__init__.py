@@ -571,7 +571,7 @@ def run_noalloc(graph, steps, params=None, dump=False, **kw):
 
 
 class Batch:
-    """Many graphs run together in one persistent launch per step."""
+    """Many graphs stepped together as one disjoint union (one launch per tier per step)."""
 
     def __init__(self, graphs, params=None, **kw):
         self.graphs = list(graphs)

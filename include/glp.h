@@ -49,7 +49,7 @@ enum {
 /* ---- opaque handles --------------------------------------------------- */
 typedef struct glp_graph glp_graph;       /* host CSR (+ device copy once used) */
 typedef struct glp_session glp_session;   /* device-resident LP state of one job */
-typedef struct glp_batch glp_batch;       /* many graphs, one persistent launch  */
+typedef struct glp_batch glp_batch;       /* many graphs stepped as one union    */
 typedef struct glp_server glp_server;     /* continuous-batching request loop    */
 
 /* ---- run parameters --------------------------------------------------- */
@@ -231,14 +231,17 @@ int glp_session_free(glp_session* s);
 int glp_run_noalloc(const glp_graph* g, const glp_params* p, int32_t steps,
                     int32_t* labels_out, int32_t* step_dump);
 
-/* ---- batched: many graphs in one persistent launch --------------------- */
+/* ---- batched: many graphs stepped as one disjoint union --------------- */
 /* No single reference counterpart: the reference runs one graph per process
  * (main.cu:18-123).  All graphs use the same params; labels are identical to
- * running each graph alone.                                                   */
+ * running each graph alone.  Every launch of a step covers all graphs; with
+ * the tuning key fused_small=1 the packed and half tiers are one persistent
+ * launch.                                                                    */
 int glp_batch_create(int32_t ngraphs, glp_graph* const* graphs,
                      const glp_params* p, glp_batch** out);
-/* mode 0: one persistent launch per step; mode 1: steps captured in a HIP
- * graph; mode 2: plain per-graph session launches (the comparison form).    */
+/* mode 0: the union's launches, enqueued per step; mode 1: the same steps
+ * replayed from a captured HIP graph; mode 2: plain per-graph session
+ * launches (the comparison form).                                          */
 int glp_batch_run(glp_batch* b, int32_t steps, int32_t mode, double* ms);
 int glp_batch_labels(glp_batch* b, int32_t graph, int32_t* out);
 int glp_batch_reset(glp_batch* b);

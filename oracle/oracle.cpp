@@ -16,9 +16,13 @@
 //                        and unreachable vertices), all-seeds identity,
 //                        invariants (seeds fixed, labels are seed ids, the
 //                        labeled set only grows and advances one hop a step)
-//   orc_slpa / _post     T=1 reduces to one classic step; stream KATs
-//   orc_countmin_step    no-collision widths reduce to classic; aliasing
-//                        closed forms; hash KATs from the constants
+//   orc_slpa / _post     T=1 reduces to one classic step; stream KATs; a
+//                        hand-traced three-step case whose speakers come
+//                        from the stream's known answers; hand-derived sets
+//   orc_countmin_step    no-collision widths reduce to classic; width-4 star
+//                        cases argued by hand from the row hashes' cells
+//                        (row 1/2 collisions decide, aliased != proper for
+//                        d = 3 and 4); hash KATs from the constants
 //   orc_table_step       capacity >= degree reduces to classic; hand traces
 //   orc_cpulp_run        mode 1 equals repeated orc_classic_step
 //   orc_classic_subset   equals orc_classic_step on every listed vertex
