@@ -6,6 +6,7 @@
             --warmup W --steps K [--gather-steps G] --workload KEY --command "..."
     python -m glp_amd.rocprof_summary sizes DIR --warmup W --steps K [--gather-steps G] --command "..."
     python -m glp_amd.rocprof_summary steps DIR --steps N --command "..."
+    python -m glp_amd.rocprof_summary counters DIR [DIR ...] [--match NAME] --command "..."
 
 `stats` condenses a --kernel-trace --stats run.  `calibrate` compares the
 FETCH_SIZE / WRITE_SIZE that rocprofv3 reports for device-to-device copies of
@@ -14,6 +15,7 @@ a known size with the bytes the copies move and writes the two scale factors.
 with one k_end_step dispatch) and reports, per kernel family, the calibrated
 HBM bytes per step over the timed steps; `traffic_for` reads that file back.
 `steps` lists a --kernel-trace run's kernel time per LP step and family.
+`counters` sums any --pmc counters per kernel instance.
 """
 import argparse
 import csv
@@ -233,6 +235,49 @@ def cmd_steps(a):
     print("\n".join(out))
 
 
+def instance(kernel_name):
+    """Kernel family with its template arguments: 'k_half<..., 4, 16, 4>'."""
+    s = kernel_name.strip().replace("(anonymous namespace)::", "")
+    if s.startswith("void "):
+        s = s[5:]
+    s = s.split("(")[0]
+    head, _, tmpl = s.partition("<")
+    name = head.split("::")[-1]
+    return name + ("<" + tmpl.replace("glp::", "") if tmpl else "")
+
+
+def cmd_counters(a):
+    """Per kernel instance, each counter summed over its dispatches (several
+    --pmc runs of one command may be given), plus derived ratios."""
+    acc = defaultdict(lambda: defaultdict(float))
+    calls = defaultdict(set)
+    for d in a.dirs:
+        for f in _csvs(d, "counter_collection.csv"):
+            with open(f) as fh:
+                for r in csv.DictReader(fh):
+                    k = instance(r["Kernel_Name"])
+                    acc[k][r["Counter_Name"]] += float(r["Counter_Value"])
+                    calls[k].add((d, r["Dispatch_Id"]))
+    names = sorted({c for v in acc.values() for c in v})
+    out = ["# rocprofv3 counters per kernel instance (sums over dispatches)",
+           "# command: %s" % a.command, "#"]
+    for k in sorted(acc, key=lambda k: -acc[k].get("SQ_WAVE_CYCLES", 0.0)):
+        if a.match and not any(m in k for m in a.match):
+            continue
+        v = acc[k]
+        out.append("%s  (dispatches %d)" % (k, len(calls[k]) // max(len(a.dirs), 1)))
+        out.append("    " + "  ".join("%s=%.4g" % (c, v[c]) for c in names if c in v))
+        if "TCC_HIT_sum" in v and v.get("TCC_HIT_sum", 0) + v.get("TCC_MISS_sum", 0) > 0:
+            out.append("    L2 hit rate %.3f" % (v["TCC_HIT_sum"] / (v["TCC_HIT_sum"] + v["TCC_MISS_sum"])))
+        if v.get("SQ_WAVE_CYCLES"):
+            for c in ("SQ_ACTIVE_INST_ANY", "SQ_WAIT_INST_ANY"):
+                if c in v:
+                    out.append("    %s / SQ_WAVE_CYCLES %.3f" % (c, v[c] / v["SQ_WAVE_CYCLES"]))
+        if v.get("SQ_INSTS_LDS"):
+            out.append("    SQ_LDS_BANK_CONFLICT / SQ_INSTS_LDS %.2f" % (v.get("SQ_LDS_BANK_CONFLICT", 0) / v["SQ_INSTS_LDS"]))
+    print("\n".join(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -264,9 +309,13 @@ def main():
     p.add_argument("--steps", type=int, required=True, help="steps of the first session to list")
     p.add_argument("--columns", type=int, default=9, help="kernel families shown (by total time)")
     p.add_argument("--command", required=True)
+    n = sub.add_parser("counters")
+    n.add_argument("dirs", nargs="+")
+    n.add_argument("--match", action="append", default=[], help="keep kernels whose name holds this")
+    n.add_argument("--command", required=True)
     a = ap.parse_args()
     {"stats": cmd_stats, "calibrate": cmd_calibrate, "traffic": cmd_traffic, "sizes": cmd_sizes,
-     "steps": cmd_steps}[a.cmd](a)
+     "steps": cmd_steps, "counters": cmd_counters}[a.cmd](a)
 
 
 if __name__ == "__main__":
