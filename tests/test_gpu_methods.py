@@ -215,7 +215,7 @@ def test_countmin_weighted(rmat13w):
 
 
 @pytest.mark.parametrize("method,buffer", [(G.FREQUENT, 0), (G.BUFFER, 1)])
-@pytest.mark.parametrize("cap", [1, 4, 16])
+@pytest.mark.parametrize("cap", [1, 4, 16, 64])
 def test_tables_match_oracle(rmat14, method, buffer, cap):
     g, (off, nbr, _) = rmat14
     _, dump, _ = G.run(g, 3, dump=True, method=method, table_cap=cap)
@@ -388,6 +388,23 @@ def test_batch_equals_alone_and_oracle(method, kw, tuned):
         b.free()
 
 
+@pytest.mark.parametrize("method,kw", [(G.CLASSIC, {}), (G.LAYERED, {"gamma": (1, 4)}),
+                                       (G.BUFFER, {"table_cap": 3})])
+def test_batch_with_tiny_and_edgeless_graphs(method, kw):
+    """Single-vertex and edge-less graphs next to ordinary ones: every mode
+    matches the oracle (an edge-less vertex keeps its label)."""
+    gs = [G.Graph.from_edges(1, []), G.Graph.rmat(9, 8, seed=4), G.Graph.from_edges(3, []),
+          G.Graph.from_edges(2, [(0, 1)]), G.Graph.from_edges(4, [(0, 1), (1, 2), (2, 3), (3, 0)])]
+    want = [oracle_labels(g, 4, method, kw) for g in gs]
+    b = G.Batch(gs, method=method, **kw)
+    for mode in (0, 1, 2):
+        b.reset()
+        b.run(4, mode=mode)
+        for i in range(len(gs)):
+            assert np.array_equal(b.labels(i), want[i]), "mode %d graph %d" % (mode, i)
+    b.free()
+
+
 def test_batch_rejects_stream_methods():
     gs = mix_graphs(2, 1)
     with pytest.raises(G.GLPError):
@@ -528,6 +545,21 @@ def test_server_budget_and_cancel():
     alone, _ = G.run(small, 2)
     assert np.array_equal(got, alone)
     assert sv.poll(a).state == 3
+    sv.close()
+
+
+def test_server_zero_steps_and_tiny_graphs():
+    """Jobs of 0 steps end with L_0; single-vertex, two-vertex and small RMAT
+    graphs served together match the oracle."""
+    sv = G.Server(budget_bytes=64 << 20, max_slots=4)
+    gs = [G.Graph.from_edges(1, []), G.Graph.from_edges(2, [(0, 1)]), G.Graph.rmat(8, 8, seed=5)]
+    jobs = [(sv.submit(g, s), g, s) for g in gs for s in (0, 3)]
+    for jid, g, s in jobs:
+        got, st = sv.wait(jid, g.n)
+        assert st.state == 2 and st.steps == s
+        off, nbr, _ = g.csr()
+        want = O.classic(off, nbr, s)[-1] if s else np.arange(g.n, dtype=np.int32)
+        assert np.array_equal(got, want), "graph n=%d steps %d" % (g.n, s)
     sv.close()
 
 
