@@ -495,6 +495,11 @@ def bench_batched(args, G):
         "batched_eager_ms": round(eager_ms, 2), "plain_ms": round(plain_ms, 2) if plain_ms else None,
         "batched_ms": round(batch_ms, 2), "labels_identical": same,
         "gteps_batched": round(m_tot * steps / (batch_ms / 1e3) / 1e9, 3),
+        # the union step's algorithmic bytes (8 B per edge, 12 B per vertex)
+        # over its replayed step time
+        "step_roofline": {"algorithmic_bytes": int(8 * m_tot + 12 * n_tot),
+                          "achieved_GBs": round((8 * m_tot + 12 * n_tot) / (batch_ms / steps / 1e3) / 1e9, 1),
+                          "frac": round((8 * m_tot + 12 * n_tot) / (batch_ms / steps / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
     }
 
 

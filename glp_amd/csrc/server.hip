@@ -14,11 +14,13 @@
 //    and one that fills the layered jobs' volumes.  Layered jobs with
 //    gamma > 0 run the slot-score hooks from a second table (one more set of
 //    tier launches), after a folded launch writes their {label, volume}
-//    records.  A tick's table goes up
-//    in one copy from a ring of three pinned staging slots; the worker waits
-//    only for the tick that last used a slot, so two ticks stay queued on
-//    the device while it prepares the next.  Jobs admitted meanwhile start
-//    (their state initialised on the tick stream) and join the next tick.
+//    records.  A step's tables go up in one copy from a ring of three
+//    pinned staging slots; the worker waits only for the step that last
+//    used a slot, so two steps stay queued on the device while it prepares
+//    the next.  Jobs admitted meanwhile start (their state initialised on a
+//    second stream, lstream) and join the next tick; those far smaller than
+//    the tick's work (light jobs) run all their steps at once on lstream,
+//    beside the heavy stream, instead of one step per tick.
 //  * the upload thread copies private graphs to the device on its own
 //    stream while the ticks run.
 //  * the completion thread waits for a finished job's last tick, copies its
@@ -206,14 +208,18 @@ struct glp_server {
     std::string worker_msg;
     std::vector<hipEvent_t> free_events;
     std::thread worker, uploader, completer;
-    hipStream_t stream = nullptr;      // ticks
+    hipStream_t stream = nullptr;      // ticks: one step of every heavy job
+    hipStream_t lstream = nullptr;     // job initialisation and light jobs' steps
+    hipEvent_t init_ev = nullptr;      // after the latest initialisations on lstream
     hipStream_t up_stream = nullptr;   // private graph uploads
     hipStream_t dl_stream = nullptr;   // result copies
     hipStream_t add_stream = nullptr;  // glp_server_add_* and graph copies (callers' threads)
     std::mutex add_mu;
     DevicePool pool;
-    TickSlot slots[kTickSlots];
+    TickSlot slots[kTickSlots];   // staging of the heavy steps (stream)
     int next_slot = 0;
+    TickSlot lslots[kTickSlots];  // staging of the light steps (lstream)
+    int next_lslot = 0;
     std::vector<ServerJob*> running;  // the worker's
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double now() const {
@@ -232,7 +238,8 @@ struct glp_server {
     void complete_loop();
     int start_job(ServerJob* j);
     int tick();
-    void hand_over(ServerJob* j);
+    int step_jobs(const std::vector<ServerJob*>& jobs, hipStream_t s, TickSlot* ring, int& next);
+    void hand_over(ServerJob* j, hipStream_t s);
     std::unique_ptr<ResidentGraph> release_graph_locked(ResidentGraph* rg);
     hipEvent_t take_event();
 };
@@ -268,7 +275,8 @@ static size_t resident_bytes(const DeviceGraph& dg, bool weighted) {
     return dg.bytes + ((size_t)dg.n + 1) * 4 + (size_t)dg.m * 4 * (weighted ? 2 : 1);
 }
 
-// Allocates and initialises a job's state on the tick stream.
+// Allocates and initialises a job's state on lstream (the heavy stream waits
+// for init_ev before its next step).
 int glp_server::start_job(ServerJob* j) {
     const DeviceGraph& dg = *j->rg->dg;
     const Plan& p = dg.plan;
@@ -288,13 +296,13 @@ int glp_server::start_job(ServerJob* j) {
         j->hs.hist = reinterpret_cast<uint32_t*>(base + lay.hist);
         j->hs.lstart = reinterpret_cast<uint32_t*>(base + lay.lstart);
         j->hs.best = reinterpret_cast<unsigned long long*>(base + lay.best);
-        if ((rc = launch_zero_bytes(j->hs.best, (int64_t)p.hubs.size() * 8, stream))) return rc;
+        if ((rc = launch_zero_bytes(j->hs.best, (int64_t)p.hubs.size() * 8, lstream))) return rc;
     }
     j->changes = reinterpret_cast<unsigned long long*>(base + lay.changes);
-    if ((rc = launch_zero_bytes(j->changes, 8 * kChangeSlots, stream))) return rc;
-    if ((rc = launch_iota(j->L[0], dg.n, stream))) return rc;
-    if ((rc = launch_iota(j->L[1], dg.n, stream))) return rc;
-    if (layered && (rc = launch_fill_i32(j->V[0], 1, dg.n, stream))) return rc;  // V_0 = 1
+    if ((rc = launch_zero_bytes(j->changes, 8 * kChangeSlots, lstream))) return rc;
+    if ((rc = launch_iota(j->L[0], dg.n, lstream))) return rc;
+    if ((rc = launch_iota(j->L[1], dg.n, lstream))) return rc;
+    if (layered && (rc = launch_fill_i32(j->V[0], 1, dg.n, lstream))) return rc;  // V_0 = 1
     j->cur = 0;
     j->done_steps = 0;
     j->start_ms = now();
@@ -302,27 +310,80 @@ int glp_server::start_job(ServerJob* j) {
     return GLP_OK;
 }
 
-// Hands a job whose last tick is enqueued (or that ends without one) to the
-// completion thread.
-void glp_server::hand_over(ServerJob* j) {
+// Hands a job whose last step is enqueued on s (or that ends without one) to
+// the completion thread.
+void glp_server::hand_over(ServerJob* j, hipStream_t s) {
     j->done_ev = take_event();
-    if (j->done_ev) (void)hipEventRecord(j->done_ev, stream);
+    if (j->done_ev) (void)hipEventRecord(j->done_ev, s);
     std::lock_guard<std::mutex> lk(mu);
     finish_q.push_back(j);
     cv_finish.notify_one();
 }
 
-// Enqueues one step of every running job with folded launches.
-//
-// Two job tables per tick: classic and gamma = 0 layered jobs run the
-// layered hooks (gamma = 0 reads no volume), layered jobs with gamma > 0 the
-// slot-score hooks, which read each neighbour's {label, volume} record
-// (written for those jobs by one folded launch before the tiers) and score
-// in the table slots.
+// The light jobs of a tick together cost at most 1/kLightShare of one step
+// of every running job (cost: vertices + edges per step).
+constexpr int64_t kLightShare = 4;
+
+// One tick.  Jobs that have not stepped yet and are far smaller than the
+// tick's work (light: taken smallest first while their whole runs sum to at
+// most 1/kLightShare of one step of every running job) run all their steps
+// now on lstream, beside the heavy stream, and are handed over at once;
+// every other job advances one step on the heavy stream.  A small job then
+// finishes within one tick instead of waiting for ten ticks whose length the
+// largest jobs set, and the heavy ticks do not grow by its launches.  A job
+// that has stepped on the heavy stream stays there (its steps are ordered
+// by that stream).
 int glp_server::tick() {
-    std::vector<ServerJob*> act;
-    for (auto* j : running)
-        if (j->done_steps < j->steps && !j->cancel) act.push_back(j);
+    std::vector<ServerJob*> light, heavy;
+    std::vector<std::pair<int64_t, ServerJob*>> fresh;
+    int64_t total = 0;
+    for (auto* j : running) {
+        if (j->done_steps >= j->steps || j->cancel) continue;
+        const int64_t per = (int64_t)j->rg->n + j->rg->m;
+        total += per;
+        if (j->done_steps == 0) fresh.emplace_back((int64_t)j->steps * per, j);
+        else heavy.push_back(j);
+    }
+    std::stable_sort(fresh.begin(), fresh.end(),
+                     [](const std::pair<int64_t, ServerJob*>& x, const std::pair<int64_t, ServerJob*>& y) {
+                         return x.first < y.first;
+                     });
+    int64_t taken = 0;
+    for (const auto& f : fresh) {
+        if ((taken + f.first) * kLightShare <= total) {
+            taken += f.first;
+            light.push_back(f.second);
+        } else {
+            heavy.push_back(f.second);
+        }
+    }
+    while (!light.empty()) {
+        int rc = step_jobs(light, lstream, lslots, next_lslot);
+        if (rc != GLP_OK) return rc;
+        for (auto it = light.begin(); it != light.end();) {
+            ServerJob* j = *it;
+            if (j->cancel || j->done_steps >= j->steps) {
+                hand_over(j, lstream);
+                running.erase(std::find(running.begin(), running.end(), j));
+                it = light.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    return heavy.empty() ? GLP_OK : step_jobs(heavy, stream, slots, next_slot);
+}
+
+// Enqueues one step of `jobs` on stream s with folded launches, staging the
+// tables through `ring`.
+//
+// Two job tables: classic and gamma = 0 layered jobs run the layered hooks
+// (gamma = 0 reads no volume), layered jobs with gamma > 0 the slot-score
+// hooks, which read each neighbour's {label, volume} record (written for
+// those jobs by one folded launch before the tiers) and score in the table
+// slots.
+int glp_server::step_jobs(const std::vector<ServerJob*>& jobs, hipStream_t s, TickSlot* ring, int& next) {
+    std::vector<ServerJob*> act = jobs;
     if (act.empty()) return GLP_OK;
     // slot-score jobs last: jobs [0, na) form table A, [na, nj) table B
     std::stable_partition(act.begin(), act.end(), [](const ServerJob* j) { return j->LV == nullptr; });
@@ -344,9 +405,9 @@ int glp_server::tick() {
     const size_t o_rec = up16(o_vc + sizeof(int32_t) * (nl + 1));
     const size_t o_rc = up16(o_rec + sizeof(VolumeJob) * std::max(nb, 1));
     const size_t total = up16(o_rc + sizeof(int32_t) * (nb + 1));
-    TickSlot& sl = slots[next_slot];
-    next_slot = (next_slot + 1) % kTickSlots;
-    if (sl.used) GLP_HIP(hipEventSynchronize(sl.ev));  // the tick that last used this slot is done
+    TickSlot& sl = ring[next];
+    next = (next + 1) % kTickSlots;
+    if (sl.used) GLP_HIP(hipEventSynchronize(sl.ev));  // the step that last used this slot is done
     if (total > sl.cap) {
         (void)hipHostFree(sl.host);
         (void)hipFree(sl.dev);
@@ -443,23 +504,23 @@ int glp_server::tick() {
     };
     prefix(o_cum_a, 0, na, grid_a);
     prefix(o_cum_b, na, nb, grid_b);
-    GLP_HIP(hipMemcpyAsync(sl.dev, sl.host, total, hipMemcpyHostToDevice, stream));
+    GLP_HIP(hipMemcpyAsync(sl.dev, sl.host, total, hipMemcpyHostToDevice, s));
     const JobDesc* d_jobs = reinterpret_cast<const JobDesc*>(sl.dev + o_jobs);
     const VolumeJob* d_vol = reinterpret_cast<const VolumeJob*>(sl.dev + o_vol);
     auto dev_i32 = [&](size_t off) { return reinterpret_cast<const int32_t*>(sl.dev + off); };
-    int rc = launch_multi_volume(kVolumeZero, d_vol, dev_i32(o_zc), nl, h_zc[nl], stream);
+    int rc = launch_multi_volume(kVolumeZero, d_vol, dev_i32(o_zc), nl, h_zc[nl], s);
     if (rc != GLP_OK) return rc;
     if ((rc = launch_multi_volume(kVolumeRecords, reinterpret_cast<const VolumeJob*>(sl.dev + o_rec),
-                                  dev_i32(o_rc), nb, h_rc[nb], stream)) != GLP_OK)
+                                  dev_i32(o_rc), nb, h_rc[nb], s)) != GLP_OK)
         return rc;
     if (na > 0 && (rc = launch_multi(weighted_a, false, d_jobs, dev_i32(o_cum_a), na, grid_a, tune, class_cap,
-                                     stream)) != GLP_OK)
+                                     s)) != GLP_OK)
         return rc;
     if (nb > 0 && (rc = launch_multi(weighted_b, true, d_jobs + na, dev_i32(o_cum_b), nb, grid_b, tune,
-                                     class_cap, stream)) != GLP_OK)
+                                     class_cap, s)) != GLP_OK)
         return rc;
-    if ((rc = launch_multi_volume(kVolumeFill, d_vol, dev_i32(o_vc), nl, h_vc[nl], stream)) != GLP_OK) return rc;
-    GLP_HIP(hipEventRecord(sl.ev, stream));
+    if ((rc = launch_multi_volume(kVolumeFill, d_vol, dev_i32(o_vc), nl, h_vc[nl], s)) != GLP_OK) return rc;
+    GLP_HIP(hipEventRecord(sl.ev, s));
     sl.used = true;
     for (auto* j : act) {
         j->cur ^= 1;
@@ -525,6 +586,7 @@ void glp_server::run() {
             uploaded.clear();
         }
         dead.clear();
+        bool started = false;
         for (auto* j : starting) {
             int rc = j->error ? j->error : start_job(j);
             if (rc != GLP_OK) {
@@ -533,10 +595,19 @@ void glp_server::run() {
                     std::lock_guard<std::mutex> lk(mu);
                     worker_msg = glp_last_error();
                 }
-                hand_over(j);
+                hand_over(j, lstream);
                 continue;
             }
             running.push_back(j);
+            started = true;
+        }
+        if (started) {
+            // the heavy stream's next step may include the jobs just
+            // initialised on lstream
+            if (hipEventRecord(init_ev, lstream) != hipSuccess || hipStreamWaitEvent(stream, init_ev, 0) != hipSuccess) {
+                std::lock_guard<std::mutex> lk(mu);
+                worker_msg = "ordering a job's initialisation before its steps failed";
+            }
         }
         int rc = tick();
         if (rc != GLP_OK) {
@@ -545,9 +616,10 @@ void glp_server::run() {
                 worker_msg = glp_last_error();
             }
             (void)hipStreamSynchronize(stream);
+            (void)hipStreamSynchronize(lstream);
             for (auto* j : running) {
                 j->error = rc;
-                hand_over(j);
+                hand_over(j, stream);
             }
             running.clear();
             continue;
@@ -555,7 +627,7 @@ void glp_server::run() {
         for (auto it = running.begin(); it != running.end();) {
             ServerJob* j = *it;
             if (j->cancel || j->done_steps >= j->steps) {
-                hand_over(j);
+                hand_over(j, stream);
                 it = running.erase(it);
             } else {
                 ++it;
@@ -582,6 +654,10 @@ void glp_server::upload_loop() {
         ResidentGraph* rg = j->rg;
         DeviceGraph* dg = nullptr;
         int rc = device_graph_upload(rg->host->csr, std::move(rg->plan), up_stream, &dg);
+        // the copies complete before the host copy is freed and before any
+        // other stream runs the job
+        if (rc == GLP_OK && hipStreamSynchronize(up_stream) != hipSuccess)
+            rc = set_error(GLP_E_HIP, "graph upload: %s", hipGetErrorString(hipGetLastError()));
         std::lock_guard<std::mutex> lk(mu);
         if (rc == GLP_OK) {
             rg->dg.reset(dg);
@@ -691,10 +767,13 @@ extern "C" int glp_server_create(uint64_t hbm_budget_bytes, int32_t max_slots, g
         for (int c = 0; c < kMaxBlockClasses; ++c) sv->class_cap[c] = p.class_cap[c];
     }
     GLP_HIP(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
+    GLP_HIP(hipStreamCreateWithFlags(&sv->lstream, hipStreamNonBlocking));
+    GLP_HIP(hipEventCreateWithFlags(&sv->init_ev, hipEventDisableTiming));
     GLP_HIP(hipStreamCreateWithFlags(&sv->up_stream, hipStreamNonBlocking));
     GLP_HIP(hipStreamCreateWithFlags(&sv->dl_stream, hipStreamNonBlocking));
     GLP_HIP(hipStreamCreateWithFlags(&sv->add_stream, hipStreamNonBlocking));
     for (auto& sl : sv->slots) GLP_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+    for (auto& sl : sv->lslots) GLP_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
     glp_server* raw = sv.get();
     sv->worker = std::thread([raw] { raw->run(); });
     sv->uploader = std::thread([raw] { raw->upload_loop(); });
@@ -735,6 +814,11 @@ extern "C" int glp_server_add_graph(glp_server* sv, const glp_graph* g, int32_t*
     {
         std::lock_guard<std::mutex> lk(sv->add_mu);
         if ((rc = device_graph_upload(c, std::move(plan), sv->add_stream, &dg)) != GLP_OK) return rc;
+        // complete before any job stream reads it
+        if (hipStreamSynchronize(sv->add_stream) != hipSuccess) {
+            delete dg;
+            return set_error(GLP_E_HIP, "graph upload: %s", hipGetErrorString(hipGetLastError()));
+        }
     }
     rg->dg.reset(dg);
     rg->bytes = dg->bytes;
@@ -984,15 +1068,19 @@ extern "C" int glp_server_free(glp_server* sv) {
     if (sv->uploader.joinable()) sv->uploader.join();
     if (sv->completer.joinable()) sv->completer.join();
     (void)hipStreamSynchronize(sv->stream);
-    for (auto& sl : sv->slots) {
-        (void)hipHostFree(sl.host);
-        (void)hipFree(sl.dev);
-        if (sl.ev) (void)hipEventDestroy(sl.ev);
-    }
+    (void)hipStreamSynchronize(sv->lstream);
+    for (TickSlot* ring : {sv->slots, sv->lslots})
+        for (int k = 0; k < kTickSlots; ++k) {
+            TickSlot& sl = ring[k];
+            (void)hipHostFree(sl.host);
+            (void)hipFree(sl.dev);
+            if (sl.ev) (void)hipEventDestroy(sl.ev);
+        }
+    if (sv->init_ev) (void)hipEventDestroy(sv->init_ev);
     for (hipEvent_t e : sv->free_events) (void)hipEventDestroy(e);
     sv->graphs.clear();
     sv->pool.trim(0);
-    for (hipStream_t s : {sv->stream, sv->up_stream, sv->dl_stream, sv->add_stream})
+    for (hipStream_t s : {sv->stream, sv->lstream, sv->up_stream, sv->dl_stream, sv->add_stream})
         if (s) (void)hipStreamDestroy(s);
     delete sv;
     return GLP_OK;
