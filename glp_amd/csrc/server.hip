@@ -322,7 +322,7 @@ void glp_server::hand_over(ServerJob* j, hipStream_t s) {
 
 // The light jobs of a tick together cost at most 1/kLightShare of one step
 // of every running job (cost: vertices + edges per step).
-constexpr int64_t kLightShare = 4;
+constexpr int64_t kLightShare = 2;
 
 // One tick.  Jobs that have not stepped yet and are far smaller than the
 // tick's work (light: taken smallest first while their whole runs sum to at
