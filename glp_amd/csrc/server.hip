@@ -767,7 +767,23 @@ extern "C" int glp_server_create(uint64_t hbm_budget_bytes, int32_t max_slots, g
         for (int c = 0; c < kMaxBlockClasses; ++c) sv->class_cap[c] = p.class_cap[c];
     }
     GLP_HIP(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
-    GLP_HIP(hipStreamCreateWithFlags(&sv->lstream, hipStreamNonBlocking));
+    {
+        // The light jobs overlap the heavy ticks only if lstream does not
+        // share the tick stream's hardware queue: streams share the
+        // process's few queues (GPU_MAX_HW_QUEUES, 4 by default), and after
+        // a process has created and destroyed many streams two new ones
+        // can land on one queue.  A stream created with a CU mask (here
+        // every CU) gets a queue of its own.
+        int dev = 0, cus = 0;
+        GLP_HIP(hipGetDevice(&dev));
+        GLP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        std::vector<uint32_t> mask((size_t)(cus + 31) / 32, 0xffffffffu);
+        if (hipExtStreamCreateWithCUMask(&sv->lstream, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
+            (void)hipGetLastError();
+            sv->lstream = nullptr;
+            GLP_HIP(hipStreamCreateWithFlags(&sv->lstream, hipStreamNonBlocking));
+        }
+    }
     GLP_HIP(hipEventCreateWithFlags(&sv->init_ev, hipEventDisableTiming));
     GLP_HIP(hipStreamCreateWithFlags(&sv->up_stream, hipStreamNonBlocking));
     GLP_HIP(hipStreamCreateWithFlags(&sv->dl_stream, hipStreamNonBlocking));
