@@ -1003,6 +1003,10 @@ __device__ __forceinline__ void part_pass(const HubArgs& h, const PartJob& j, ui
         const uint32_t s_c = c < j.nch ? c * (uint32_t)h.chunk + st[c] : 0u;
         const int32_t incl = wave_incl_sum((int32_t)n_c);
         const uint32_t wtot = (uint32_t)__builtin_amdgcn_readfirstlane(shfl_i32(incl, 63));
+        // the window's runs: lanes [0, nr); the owner search needs
+        // ceil(log2(nr)) halvings (most hubs have a few chunks)
+        const int nr = (int)min(64u, j.nch - c0);
+        const int depth = nr > 1 ? 32 - __builtin_clz((uint32_t)(nr - 1)) : 0;
         // the trip count is wave-uniform, so the wave reductions below run
         // with every lane; lanes past the window's pairs load a clamped pair
         // and skip it
@@ -1016,9 +1020,9 @@ __device__ __forceinline__ void part_pass(const HubArgs& h, const PartJob& j, ui
                 in[u] = f < wtot;
                 const int32_t fc = (int32_t)min(f, wtot - 1u);
                 // owner: the first lane whose inclusive sum exceeds fc
-                int o = 0, top = 63;
-#pragma unroll
-                for (int k = 0; k < 6; ++k) {
+                // (lanes >= nr hold no pairs, so it lies in [0, nr))
+                int o = 0, top = nr - 1;
+                for (int k = 0; k < depth; ++k) {  // wave-uniform trip count
                     const int mid = (o + top) >> 1;
                     if (shfl_i32(incl, mid) <= fc) o = mid + 1;
                     else top = mid;
