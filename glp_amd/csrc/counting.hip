@@ -130,6 +130,24 @@ __global__ __launch_bounds__(NT) void k_countmin(StepArgs a, const int32_t* __re
     }
 }
 
+// Min over the 64 lanes, every lane active, without the LDS pipe: DPP moves
+// within each row of 16 (quad pairs, quad with mirror quad, half-row with
+// half-row), then the four rows' minima by readlane.  A lane whose DPP
+// source is unavailable keeps its own value (old = x).  The full-table
+// case of k_table_wave takes one per edge, in series, so its latency sets
+// the wide tables' speed.
+__device__ __forceinline__ uint32_t wave_min_dpp(uint32_t x) {
+    x = min(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0xB1, 0xf, 0xf, false));   // quad_perm [1,0,3,2]
+    x = min(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x4E, 0xf, 0xf, false));   // quad_perm [2,3,0,1]
+    x = min(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x141, 0xf, 0xf, false));  // row half-mirror
+    x = min(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x140, 0xf, 0xf, false));  // row mirror
+    const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)x, 0);
+    const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)x, 16);
+    const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)x, 32);
+    const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane((int)x, 48);
+    return min(min(r0, r1), min(r2, r3));
+}
+
 // One wavefront per vertex of `verts` (degrees above kTableLaneMaxDeg):
 // slot i of the vertex's table is lane i's (K, C) (lanes >= cap hold no
 // slot).  Edges are loaded 64 at a time and applied one after another in
@@ -170,7 +188,7 @@ __global__ __launch_bounds__(256) void k_table_wave(StepArgs a, const int32_t* _
             // full: every counter drops (buffer: by the frequency; frequent:
             // by min(frequency, smallest counter)), emptied slots are freed
             uint32_t d = ff;
-            if (!BUF) d = min(ff, wave_min_u32(slot ? C : 0xffffffffu));
+            if (!BUF) d = min(ff, wave_min_dpp(slot ? C : 0xffffffffu));
             if (slot) C = BUF ? (C > ff ? C - ff : 0u) : C - d;
             const bool emptied = slot && C == 0u;
             if (emptied) K = 0u;
