@@ -26,9 +26,11 @@
 // approximate.  The winner is the largest (count, label) left in the table;
 // a vertex whose table ends empty keeps its label.
 //
-// Degree tiers.  Count-min: one wavefront per vertex up to
-// kCountminWaveMaxDeg edges, one 1024-thread workgroup per vertex above (the
-// sketch is shared by the workgroup).  Frequent and buffer: one lane per
+// Degree tiers.  Count-min: 16 lanes per vertex up to kCountminSmallDeg
+// edges, with no sketch (each edge's estimate is summed over the vertex's
+// edges whose cells coincide with its own, which is what the sketch cell
+// would hold); one wavefront per vertex up to kCountminWaveMaxDeg edges and
+// one 1024-thread workgroup per vertex above, with the sketch in LDS.  Frequent and buffer: one lane per
 // vertex up to kTableLaneMaxDeg edges (the lane walks its table slot by
 // slot), one wavefront per vertex above, with table slot i held by lane i
 // and each edge applied by the whole wavefront in CSR order: the order the
@@ -67,34 +69,110 @@ __device__ __forceinline__ uint32_t cm_hash(int r, uint32_t k) {
     }
 }
 
-// Cells (row*W + column) a key touches, one entry per increment.
-__device__ __forceinline__ int cm_cells(uint32_t key, int rows, int W, int aliased, uint32_t* cell) {
-    if (!aliased || rows <= 2 || rows == 5) {
-        for (int r = 0; r < rows; ++r) cell[r] = (uint32_t)r * W + cm_hash(r, key) % W;
-        return rows;
+// Cells (row*W + column) a key touches, one entry per increment; the first
+// `rows` entries are used.  All five are computed with constant indices so
+// the array stays in registers.  Aliased rows (d = 3 or 4): row 3's hash
+// lands in row 2, and for d = 4 h3 goes into row 2 again.
+// x mod W by Lemire, Kaser and Kurz's direct remainder ("Faster remainder
+// by direct computation", 2019): with M = floor((2^64 - 1) / W) + 1, the
+// remainder of any 32-bit x is the high 64 bits of (M x mod 2^64) * W.
+__device__ __forceinline__ uint32_t cm_mod(uint32_t x, uint64_t M, uint32_t W) {
+    return (uint32_t)__umul64hi(M * (uint64_t)x, (uint64_t)W);
+}
+__host__ __device__ inline uint64_t cm_mod_magic(uint32_t W) { return ~0ull / W + 1ull; }
+
+__device__ __forceinline__ int cm_cells(uint32_t key, int rows, int W, int aliased, uint64_t M, uint32_t* cell) {
+#pragma unroll
+    for (int r = 0; r < 5; ++r) cell[r] = (uint32_t)r * W + cm_mod(cm_hash(r, key), M, (uint32_t)W);
+    if (aliased && (rows == 3 || rows == 4)) {
+        cell[2] -= (uint32_t)W;
+        cell[3] = cell[2];
     }
-    cell[0] = cm_hash(0, key) % W;
-    cell[1] = (uint32_t)W + cm_hash(1, key) % W;
-    cell[2] = (uint32_t)W + cm_hash(2, key) % W;  // row 3's hash lands in row 2
-    if (rows == 3) return 3;
-    cell[3] = cell[2];                             // d=4: h3 into row 2 again
-    return 4;
+    return rows;
 }
 
+constexpr int32_t kCountminSmallDeg = 16;
 constexpr int32_t kCountminWaveMaxDeg = 1024;
 constexpr int32_t kTableLaneMaxDeg = 32;
 
-// NT = 64: vertex blockIdx.x, skipped when its degree is above
-// kCountminWaveMaxDeg; NT = 1024: vertex verts[blockIdx.x].
+// Vertices of degree <= kCountminSmallDeg, 16 lanes each (lane i holds edge
+// i); the others are skipped (the list launches take them).  The sketch
+// cell an entry c of edge i lands in ends up holding the sum, over the
+// vertex's edges j and their entries, of f_j where the entry is c; the
+// estimate is the minimum of that over edge i's entries.
+template <bool W>
+__global__ __launch_bounds__(256) void k_countmin_small(StepArgs a, int32_t rows, int32_t width, int32_t aliased,
+                                                       uint64_t M) {
+    const int lane = lane_id(), ql = lane & 15, g0 = lane & ~15;
+    const int64_t vv = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
+    const bool in = vv < a.n;
+    const int32_t v = in ? (int32_t)vv : 0;
+    int32_t b = 0, e = 0;
+    if (in) {
+        b = a.off[v];
+        e = a.off[v + 1];
+    }
+    const bool mine = in && e - b <= kCountminSmallDeg;  // the group's vertex is counted here
+    const bool live = mine && ql < e - b;
+    uint32_t key = 0u, f = 0u;  // a lane without an edge adds nothing (f = 0)
+    if (live) {
+        key = (uint32_t)a.L[a.nbr[b + ql]] + 1u;
+        f = W ? a.w[b + ql] : 1u;
+    }
+    uint32_t c[5];
+    const int k = cm_cells(key, rows, width, aliased, M, c);  // k depends on rows and aliased only
+    // Proper rows occupy disjoint bands [r W, (r+1) W), so entry t of one
+    // edge can only meet entry t of another; aliased rows put entries 1..3
+    // in one band, so those are compared crosswise.  (Loops unrolled to the
+    // 5 entries and guarded by k, so the arrays stay in registers.)
+    const bool cross = aliased && (rows == 3 || rows == 4);
+    // edges of the wave's groups: the loop runs to the largest degree among them
+    const int32_t dmax = __builtin_amdgcn_readfirstlane(
+        (int32_t)wave_max_u64(live ? (uint64_t)(e - b) : 0ull));
+    uint32_t sum[5] = {0u, 0u, 0u, 0u, 0u};
+    for (int jj = 0; jj < dmax; ++jj) {
+        const uint32_t fj = shfl_u32(f, g0 + jj);
+        uint32_t cj[5];
+#pragma unroll
+        for (int t = 0; t < 5; ++t) cj[t] = t < k ? shfl_u32(c[t], g0 + jj) : 0xffffffffu;
+        if (!cross) {
+#pragma unroll
+            for (int t = 0; t < 5; ++t) sum[t] += t < k && cj[t] == c[t] ? fj : 0u;
+        } else {
+#pragma unroll
+            for (int t = 0; t < 5; ++t)
+#pragma unroll
+                for (int t2 = 0; t2 < 5; ++t2) sum[t] += t < k && t2 < k && cj[t2] == c[t] ? fj : 0u;
+        }
+    }
+    uint32_t est = 0xffffffffu;
+#pragma unroll
+    for (int t = 0; t < 5; ++t)
+        if (t < k) est = min(est, sum[t]);
+    uint64_t best = live ? pack_score((int32_t)min(est, 0x7fffffffu), key) : 0ull;
+    for (int d = 8; d >= 1; d >>= 1) best = max_u64(best, shfl_u64(best, lane ^ d));
+    bool changed = false;
+    if (mine && ql == 0) {
+        const int32_t old = a.keep[v];
+        const int32_t nl = best ? packed_label(best) : old;
+        a.Lw[v] = nl;
+        changed = nl != old;
+    }
+    const unsigned long long bal = __ballot(changed);
+    if (lane == 0 && bal)
+        atomicAdd(&a.changes[blockIdx.x & (kChangeSlots - 1)], (unsigned long long)__popcll(bal));
+}
+
+// Vertex verts[blockIdx.x], the sketch in LDS (NT = 64: degrees up to
+// kCountminWaveMaxDeg; NT = 1024: above).
 template <bool W, int NT>
 __global__ __launch_bounds__(NT) void k_countmin(StepArgs a, const int32_t* __restrict__ verts, int32_t rows,
-                                                 int32_t width, int32_t aliased) {
+                                                 int32_t width, int32_t aliased, uint64_t M) {
     extern __shared__ uint32_t cells[];
     __shared__ unsigned long long s_best[NT / 64];
     const int lane = lane_id();
-    const int32_t v = NT > 64 ? verts[blockIdx.x] : (int32_t)blockIdx.x;
+    const int32_t v = verts[blockIdx.x];
     const int32_t b = a.off[v], e = a.off[v + 1];
-    if (NT == 64 && e - b > kCountminWaveMaxDeg) return;  // workgroup-uniform: the wide launch takes it
     const int32_t ncell = rows * width;
     for (int32_t i = threadIdx.x; i < ncell; i += NT) cells[i] = 0u;
     __syncthreads();
@@ -102,17 +180,21 @@ __global__ __launch_bounds__(NT) void k_countmin(StepArgs a, const int32_t* __re
         const uint32_t key = (uint32_t)a.L[a.nbr[j]] + 1u;
         const uint32_t f = W ? a.w[j] : 1u;
         uint32_t c[5];
-        const int k = cm_cells(key, rows, width, aliased, c);
-        for (int i = 0; i < k; ++i) atomicAdd(&cells[c[i]], f);
+        const int k = cm_cells(key, rows, width, aliased, M, c);
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+            if (i < k) atomicAdd(&cells[c[i]], f);
     }
     __syncthreads();
     uint64_t best = 0ull;
     for (int32_t j = b + (int32_t)threadIdx.x; j < e; j += NT) {
         const uint32_t key = (uint32_t)a.L[a.nbr[j]] + 1u;
         uint32_t c[5];
-        const int k = cm_cells(key, rows, width, aliased, c);
+        const int k = cm_cells(key, rows, width, aliased, M, c);
         uint32_t est = 0xffffffffu;
-        for (int i = 0; i < k; ++i) est = min(est, cells[c[i]]);
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+            if (i < k) est = min(est, cells[c[i]]);
         best = max_u64(best, pack_score((int32_t)min(est, 0x7fffffffu), key));
     }
     best = wave_max_u64(best);
@@ -272,15 +354,23 @@ void init_counting_kernels() {
                               lim);
 }
 
-// The vertices of degree above `max_deg`, for the counting methods' wide
-// launches (built once per session).
+// The counting methods' vertex lists (built once per session), in one
+// array: count-min, the vertices of degree in (kCountminSmallDeg,
+// kCountminWaveMaxDeg] (nmid of them), then those above (nwide); frequent
+// and buffer, the vertices above kTableLaneMaxDeg (nwide).
 int counting_wide_list(glp_session* s) {
     const HostCSR& g = s->g->csr;
-    const int32_t lim = s->p.method == GLP_COUNTMIN ? kCountminWaveMaxDeg : kTableLaneMaxDeg;
-    std::vector<int32_t> wide;
-    for (int32_t v = 0; v < g.n; ++v)
-        if (g.offsets[v + 1] - g.offsets[v] > lim) wide.push_back(v);
-    s->nwide = (int32_t)wide.size();
+    const bool cm = s->p.method == GLP_COUNTMIN;
+    const int32_t lim = cm ? kCountminWaveMaxDeg : kTableLaneMaxDeg;
+    std::vector<int32_t> wide, big;
+    for (int32_t v = 0; v < g.n; ++v) {
+        const int32_t d = g.offsets[v + 1] - g.offsets[v];
+        if (d > lim) big.push_back(v);
+        else if (cm && d > kCountminSmallDeg) wide.push_back(v);
+    }
+    s->nmid = (int32_t)wide.size();
+    s->nwide = (int32_t)big.size();
+    wide.insert(wide.end(), big.begin(), big.end());
     if (wide.empty()) return GLP_OK;
     GLP_HIP(hipMalloc(&s->wide, wide.size() * 4));
     GLP_HIP(hipMemcpyAsync(s->wide, wide.data(), wide.size() * 4, hipMemcpyHostToDevice, s->stream));
@@ -306,15 +396,21 @@ int enqueue_counting_step(glp_session* s) {
         if (s->p.method == GLP_COUNTMIN) {
             const size_t lds = (size_t)s->p.cm_rows * s->p.cm_width * 4;
             const int32_t r = s->p.cm_rows, wd = s->p.cm_width, al = s->p.cm_aliased;
-            if (w) hipLaunchKernelGGL((k_countmin<true, 64>), dim3(s->n), dim3(64), lds, s->stream, a,
-                                      nullptr, r, wd, al);
-            else hipLaunchKernelGGL((k_countmin<false, 64>), dim3(s->n), dim3(64), lds, s->stream, a,
-                                    nullptr, r, wd, al);
+            const uint64_t M = cm_mod_magic((uint32_t)wd);
+            const unsigned sgrid = (unsigned)((s->n + 15) / 16);
+            if (w) hipLaunchKernelGGL((k_countmin_small<true>), dim3(sgrid), dim3(256), 0, s->stream, a, r, wd, al, M);
+            else hipLaunchKernelGGL((k_countmin_small<false>), dim3(sgrid), dim3(256), 0, s->stream, a, r, wd, al, M);
+            if (s->nmid > 0) {
+                if (w) hipLaunchKernelGGL((k_countmin<true, 64>), dim3(s->nmid), dim3(64), lds, s->stream, a,
+                                          s->wide, r, wd, al, M);
+                else hipLaunchKernelGGL((k_countmin<false, 64>), dim3(s->nmid), dim3(64), lds, s->stream, a,
+                                        s->wide, r, wd, al, M);
+            }
             if (s->nwide > 0) {
                 if (w) hipLaunchKernelGGL((k_countmin<true, 1024>), dim3(s->nwide), dim3(1024), lds, s->stream,
-                                          a, s->wide, r, wd, al);
+                                          a, s->wide + s->nmid, r, wd, al, M);
                 else hipLaunchKernelGGL((k_countmin<false, 1024>), dim3(s->nwide), dim3(1024), lds,
-                                        s->stream, a, s->wide, r, wd, al);
+                                        s->stream, a, s->wide + s->nmid, r, wd, al, M);
             }
         } else {
             const size_t lds = (size_t)64 * 2 * s->p.table_cap * 4;

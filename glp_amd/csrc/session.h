@@ -23,8 +23,8 @@ struct glp_session {
     int32_t* adj = nullptr;              // pre-gathered neighbour labels (gather mode)
     int32_t* adj_alt = nullptr;          // the second adj of the two-step lookahead
     int32_t* mem = nullptr;              // SLPA memory (T+1) x n
-    int32_t* wide = nullptr;             // count-min / frequent / buffer: vertices above the narrow kernel's degree
-    int32_t nwide = 0;
+    int32_t* wide = nullptr;             // counting methods' vertex lists (count-min: nmid, then nwide)
+    int32_t nmid = 0, nwide = 0;
     uint8_t* act[2] = {nullptr, nullptr};  // frontier flags, double-buffered like L
     int32_t* fctl = nullptr;             // frontier control {all_active, mark, threshold, gather words}
     int64_t act_bytes = 0;
