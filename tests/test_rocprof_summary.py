@@ -99,3 +99,29 @@ def test_steps_table(tmp_path):
     assert [float(x) for x in body[0][1:3]] == [2.0, 1.0]
     assert [float(x) for x in body[1][1:3]] == [3.0, 0.5]
     assert float(body[1][-1]) == pytest.approx(3.501)
+
+
+def test_instance_names_and_anonymous_namespace():
+    assert RS.instance("void glp::k_half<glp::ClassicHooks, false, false, 4, 16, 4>(glp::StepArgs)") == \
+        "k_half<ClassicHooks, false, false, 4, 16, 4>"
+    assert RS.instance("glp::k_end_step(unsigned long long*)") == "k_end_step"
+    assert RS.family("glp::(anonymous namespace)::k_key_offsets(unsigned long long const*, long)") == \
+        "k_key_offsets"
+
+
+def test_counters_sums_per_instance(tmp_path):
+    a, b = str(tmp_path / "a"), str(tmp_path / "b")
+    write_counters(a, "SQ_INSTS_LDS", [("void glp::k_half<H, 4>(x)", 10.0), ("void glp::k_half<H, 4>(x)", 30.0),
+                                       ("void glp::k_packed<H, 4>(x)", 5.0)])
+    write_counters(b, "TCC_HIT_sum", [("void glp::k_half<H, 4>(x)", 3.0), ("void glp::k_half<H, 4>(x)", 3.0),
+                                      ("void glp::k_packed<H, 4>(x)", 1.0)])
+    # the miss counter in a third file of the same run as b
+    with open(os.path.join(b, "run_counter_collection.csv"), "a", newline="") as fh:
+        w = csv.writer(fh)
+        w.writerow([1, "void glp::k_half<H, 4>(x)", "TCC_MISS_sum", 2.0])
+        w.writerow([2, "void glp::k_half<H, 4>(x)", "TCC_MISS_sum", 2.0])
+    out = run_tool("counters", a, b, "--match", "k_half", "--command", "cmd")
+    assert "k_packed" not in out
+    assert "k_half<H, 4>  (dispatches 2)" in out
+    assert "SQ_INSTS_LDS=40" in out and "TCC_HIT_sum=6" in out
+    assert "L2 hit rate 0.600" in out
