@@ -604,9 +604,10 @@ void glp_server::run() {
         if (started) {
             // the heavy stream's next step may include the jobs just
             // initialised on lstream
-            if (hipEventRecord(init_ev, lstream) != hipSuccess || hipStreamWaitEvent(stream, init_ev, 0) != hipSuccess) {
-                std::lock_guard<std::mutex> lk(mu);
-                worker_msg = "ordering a job's initialisation before its steps failed";
+            if (hipEventRecord(init_ev, lstream) != hipSuccess ||
+                hipStreamWaitEvent(stream, init_ev, 0) != hipSuccess) {
+                (void)hipGetLastError();
+                (void)hipStreamSynchronize(lstream);  // order it from the host instead
             }
         }
         int rc = tick();
