@@ -64,6 +64,12 @@ def test_key_value_form_and_other_methods(graph_file, tmp_path):
     for _ in range(2):
         prev = O.countmin_step(off, nbr, prev, 3, 64, 1)
     assert np.array_equal(np.fromfile(lab, dtype=np.int32), prev)
+    for method, buf in ((6, 0), (7, 1)):   # frequent and buffer, 5 slots
+        gpulp(method, 3, graph_file, out, 0, "cap=5", "labels=%s" % lab)
+        prev = np.arange(len(off) - 1, dtype=np.int32)
+        for _ in range(2):
+            prev = O.table_step(off, nbr, prev, 5, buf)
+        assert np.array_equal(np.fromfile(lab, dtype=np.int32), prev), "method %d" % method
     lines = out.read_text().strip().splitlines()
     assert re.match(r"graph: .* method: 2 iteration: 3 time: [0-9.e-]+$", lines[1])
 
