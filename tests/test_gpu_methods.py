@@ -141,6 +141,15 @@ def test_slpa_memory_and_post(rmat14, T):
     assert np.array_equal(labels, dw)
 
 
+def test_slpa_weighted(rmat13w):
+    """Weighted SLPA: listeners sum the shown labels' edge weights."""
+    g, (off, nbr, w) = rmat13w
+    with G.Session(g, method=G.SLPA, seed=5, slpa_T=6, weighted=True) as s:
+        s.step(6)
+        mem = s.slpa_memory(7)
+    assert_steps_equal(mem, O.slpa(off, nbr, seed=5, T=6, w=w), "slpa weighted")
+
+
 def test_slpa_hand_trace():
     """The hand-traced three-step SLPA case of the oracle's tests (seed 3)."""
     g = G.Graph.from_edges(5, [(0, 1), (1, 2), (2, 3), (3, 4), (1, 3)])
