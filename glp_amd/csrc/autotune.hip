@@ -132,8 +132,7 @@ extern "C" int glp_autotune(glp_graph* g, const glp_params* p, int32_t steps, co
     if (out_path) {
         std::ofstream out(out_path);
         if (!out) return set_error(GLP_E_IO, "cannot write '%s'", out_path);
-        for (const char* k : {"pack_max_deg", "half_max_deg", "wave_max_deg", "block_max_deg", "hub_chunk", "part_keys",
-                              "part_limit", "block_threads", "fused_small"}) {
+        for (int i = 0; const char* k = tuning_key(i); ++i) {
             int32_t v = 0;
             tuning_get(cur, k, &v);
             out << k << "=" << v << "\n";

@@ -35,6 +35,7 @@ struct Tuning {
 };
 int tuning_set(Tuning& t, const char* key, int32_t value);  // GLP_E_ARG on unknown key
 int tuning_get(const Tuning& t, const char* key, int32_t* value);
+const char* tuning_key(int i);  // i-th tuning key, nullptr past the last
 Tuning& default_tuning();  // process-wide defaults (GLP_TUNING env file overrides)
 
 constexpr int kChangeSlots = 1024;  // change counters, spread to avoid one hot address

@@ -43,6 +43,10 @@ int tuning_set(Tuning& t, const char* key, int32_t value) {
     return set_error(GLP_E_ARG, "unknown tuning key '%s'", key ? key : "(null)");
 }
 
+const char* tuning_key(int i) {
+    return i >= 0 && i < (int)(sizeof(kFields) / sizeof(kFields[0])) ? kFields[i].name : nullptr;
+}
+
 int tuning_get(const Tuning& t, const char* key, int32_t* value) {
     for (const auto& f : kFields)
         if (key && std::string(key) == f.name) {

@@ -90,7 +90,10 @@ def test_autotune_keeps_results(tmp_path):
         best = G.autotune(g, p, 3, str(tmp_path / "t.tuning"))
         assert best > 0
         text = (tmp_path / "t.tuning").read_text()
-        assert "block_max_deg=" in text
+        written = dict(line.split("=") for line in text.split())
+        # every key is written, with the tuning the autotuner left in force
+        assert sorted(written) == sorted(G.TUNING_KEYS)
+        assert all(int(written[k]) == G.tuning_get(k) for k in written)
         labels, _ = G.run(g, 4)
         assert np.array_equal(labels, O.classic(off, nbr, 4)[-1])
     finally:
