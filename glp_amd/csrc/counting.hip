@@ -231,35 +231,48 @@ __device__ __forceinline__ uint32_t wave_min_dpp(uint32_t x) {
 }
 
 // One wavefront per vertex of `verts` (degrees above kTableLaneMaxDeg):
-// slot i of the vertex's table is lane i's (K, C) (lanes >= cap hold no
-// slot).  Edges are loaded 64 at a time and applied one after another in
-// CSR order; a table's keys are distinct, so at most one lane hits.
+// slot i of the vertex's table is lane i's (K, C); lanes >= cap hold the key
+// 0xffffffff, which no edge carries (keys are labels + 1 <= 2^31), so they
+// never match and are never free.  Edges are applied one after another in
+// CSR order; a table's keys are distinct, so at most one lane hits.  The
+// vertex and its edge range are wave-uniform (scalar loop control), and
+// edges come in chunks of 64, one per lane: while chunk c is applied, chunk
+// c+1's labels and chunk c+2's neighbour ids are in flight.  Lanes past the
+// last edge load the last edge again (in bounds, never applied), so the
+// loads need no branches and stay in flight across the chunk.
 template <bool W, bool BUF>
 __global__ __launch_bounds__(256) void k_table_wave(StepArgs a, const int32_t* __restrict__ verts, int32_t nv,
                                                     int32_t cap) {
     const int lane = lane_id();
     const int32_t i = (int32_t)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (i >= nv) return;  // wave-uniform
-    const int32_t v = verts[i];
-    const int32_t b = a.off[v], e = a.off[v + 1];
+    const int32_t v = __builtin_amdgcn_readfirstlane(verts[i]);
+    const int32_t b = __builtin_amdgcn_readfirstlane(a.off[v]);
+    const int32_t e = __builtin_amdgcn_readfirstlane(a.off[v + 1]);
     const bool slot = lane < cap;
-    uint32_t K = 0u, C = 0u;
+    uint32_t K = slot ? 0u : 0xffffffffu, C = 0u;
+    const int32_t last = e - 1;  // e > b: these vertices have more than 32 edges
+    int32_t jn = b + lane;
+    int32_t id = a.nbr[min(jn, last)];
+    uint32_t lab = (uint32_t)a.L[id];
+    uint32_t f = W ? a.w[min(jn, last)] : 1u;
+    jn += 64;
+    id = a.nbr[min(jn, last)];
     for (int32_t j0 = b; j0 < e; j0 += 64) {
-        const int32_t j = j0 + lane;
-        uint32_t key = 0u, f = 0u;
-        if (j < e) {
-            key = (uint32_t)a.L[a.nbr[j]] + 1u;
-            f = W ? a.w[j] : 1u;
-        }
+        const uint32_t nlab = (uint32_t)a.L[id];
+        const uint32_t nf = W ? a.w[min(jn, last)] : 1u;
+        jn += 64;
+        id = a.nbr[min(jn, last)];
         const int32_t cnt = min(64, e - j0);
         for (int32_t k = 0; k < cnt; ++k) {
-            const uint32_t kk = (uint32_t)__builtin_amdgcn_readlane((int)key, k);
+            const uint32_t kk = (uint32_t)__builtin_amdgcn_readlane((int)lab, k) + 1u;
             const uint32_t ff = W ? (uint32_t)__builtin_amdgcn_readlane((int)f, k) : 1u;
-            if (__ballot(slot && K == kk)) {  // present: it gains the frequency
-                if (slot && K == kk) C += ff;
+            const bool hit = K == kk;
+            if (__ballot(hit)) {  // present: it gains the frequency
+                if (hit) C += ff;
                 continue;
             }
-            const uint64_t fr = __ballot(slot && K == 0u);
+            const uint64_t fr = __ballot(K == 0u);
             if (fr) {  // the first free slot takes it
                 if (lane == __builtin_ctzll(fr)) {
                     K = kk;
@@ -268,7 +281,14 @@ __global__ __launch_bounds__(256) void k_table_wave(StepArgs a, const int32_t* _
                 continue;
             }
             // full: every counter drops (buffer: by the frequency; frequent:
-            // by min(frequency, smallest counter)), emptied slots are freed
+            // by min(frequency, smallest counter)), emptied slots are freed.
+            // Unweighted, every count is >= 1 and the frequency 1, so both
+            // drop by one and nothing is left to enter.
+            if (!W) {
+                if (slot) C -= 1u;
+                if (slot && C == 0u) K = 0u;
+                continue;
+            }
             uint32_t d = ff;
             if (!BUF) d = min(ff, wave_min_dpp(slot ? C : 0xffffffffu));
             if (slot) C = BUF ? (C > ff ? C - ff : 0u) : C - d;
@@ -281,6 +301,8 @@ __global__ __launch_bounds__(256) void k_table_wave(StepArgs a, const int32_t* _
                 }
             }
         }
+        lab = nlab;
+        f = nf;
     }
     const uint64_t best = wave_max_u64(slot && K ? pack_score((int32_t)min(C, 0x7fffffffu), K) : 0ull);
     if (lane == 0) {

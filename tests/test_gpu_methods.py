@@ -234,13 +234,14 @@ def test_tables_match_oracle(rmat14, method, buffer, cap):
         assert np.array_equal(dump[t], prev), "step %d" % (t + 1)
 
 
-def test_tables_weighted(rmat13w):
+@pytest.mark.parametrize("cap", [3, 16, 64])
+def test_tables_weighted(rmat13w, cap):
     g, (off, nbr, w) = rmat13w
     for method, buf in ((G.FREQUENT, 0), (G.BUFFER, 1)):
-        labels, _ = G.run(g, 2, method=method, table_cap=3, weighted=True)
+        labels, _ = G.run(g, 2, method=method, table_cap=cap, weighted=True)
         prev = np.arange(len(off) - 1, dtype=np.int32)
         for _ in range(2):
-            prev = O.table_step(off, nbr, prev, 3, buf, w=w)
+            prev = O.table_step(off, nbr, prev, cap, buf, w=w)
         assert np.array_equal(labels, prev)
 
 
