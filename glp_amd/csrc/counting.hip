@@ -30,7 +30,11 @@
 // edges, with no sketch (each edge's estimate is summed over the vertex's
 // edges whose cells coincide with its own, which is what the sketch cell
 // would hold); one wavefront per vertex up to kCountminWaveMaxDeg edges and
-// one 1024-thread workgroup per vertex above, with the sketch in LDS.  Frequent and buffer: one lane per
+// one 1024-thread workgroup per vertex up to kCountminHubDeg, with the sketch
+// in LDS; above that, chunks of kCountminChunk edges on workgroups of their
+// own (one pass adds each chunk's LDS sketch into the vertex's sketch in HBM,
+// a second estimates every edge's label against the whole sketch and keeps
+// the packed maximum with a 64-bit atomic max, a third commits).  Frequent and buffer: one lane per
 // vertex up to kTableLaneMaxDeg edges (the lane walks its table slot by
 // slot), one wavefront per vertex above, with table slot i held by lane i
 // and each edge applied by the whole wavefront in CSR order: the order the
@@ -93,27 +97,28 @@ __device__ __forceinline__ int cm_cells(uint32_t key, int rows, int W, int alias
 
 constexpr int32_t kCountminSmallDeg = 16;
 constexpr int32_t kCountminWaveMaxDeg = 1024;
+constexpr int32_t kCountminHubDeg = 8192;
+constexpr int32_t kCountminChunk = 8192;  // 1024 threads x 8 edges
 constexpr int32_t kTableLaneMaxDeg = 32;
 
-// Vertices of degree <= kCountminSmallDeg, 16 lanes each (lane i holds edge
-// i); the others are skipped (the list launches take them).  The sketch
-// cell an entry c of edge i lands in ends up holding the sum, over the
-// vertex's edges j and their entries, of f_j where the entry is c; the
-// estimate is the minimum of that over edge i's entries.
+// The vertices of `verts` (degrees 1..kCountminSmallDeg), 16 lanes each
+// (lane i holds edge i).  The sketch cell an entry c of edge i lands in ends
+// up holding the sum, over the vertex's edges j and their entries, of f_j
+// where the entry is c; the estimate is the minimum of that over edge i's
+// entries.
 template <bool W>
-__global__ __launch_bounds__(256) void k_countmin_small(StepArgs a, int32_t rows, int32_t width, int32_t aliased,
-                                                       uint64_t M) {
+__global__ __launch_bounds__(256) void k_countmin_small(StepArgs a, const int32_t* __restrict__ verts, int32_t nv,
+                                                       int32_t rows, int32_t width, int32_t aliased, uint64_t M) {
     const int lane = lane_id(), ql = lane & 15, g0 = lane & ~15;
     const int64_t vv = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
-    const bool in = vv < a.n;
-    const int32_t v = in ? (int32_t)vv : 0;
+    const bool in = vv < nv;
+    const int32_t v = in ? verts[vv] : 0;
     int32_t b = 0, e = 0;
     if (in) {
         b = a.off[v];
         e = a.off[v + 1];
     }
-    const bool mine = in && e - b <= kCountminSmallDeg;  // the group's vertex is counted here
-    const bool live = mine && ql < e - b;
+    const bool live = in && ql < e - b;
     uint32_t key = 0u, f = 0u;  // a lane without an edge adds nothing (f = 0)
     if (live) {
         key = (uint32_t)a.L[a.nbr[b + ql]] + 1u;
@@ -152,7 +157,7 @@ __global__ __launch_bounds__(256) void k_countmin_small(StepArgs a, int32_t rows
     uint64_t best = live ? pack_score((int32_t)min(est, 0x7fffffffu), key) : 0ull;
     for (int d = 8; d >= 1; d >>= 1) best = max_u64(best, shfl_u64(best, lane ^ d));
     bool changed = false;
-    if (mine && ql == 0) {
+    if (in && ql == 0) {
         const int32_t old = a.keep[v];
         const int32_t nl = best ? packed_label(best) : old;
         a.Lw[v] = nl;
@@ -163,39 +168,85 @@ __global__ __launch_bounds__(256) void k_countmin_small(StepArgs a, int32_t rows
         atomicAdd(&a.changes[blockIdx.x & (kChangeSlots - 1)], (unsigned long long)__popcll(bal));
 }
 
+// Vertices without edges keep their label.
+__global__ __launch_bounds__(256) void k_countmin_keep(StepArgs a, const int32_t* __restrict__ verts, int32_t nv) {
+    const int32_t i = (int32_t)(blockIdx.x * 256 + threadIdx.x);
+    if (i < nv) {
+        const int32_t v = verts[i];
+        a.Lw[v] = a.keep[v];
+    }
+}
+
+// Up to kCmBatch edges of a thread (j, j + NT, ...) loaded at once: the
+// indices past the last edge are clamped to it (every vertex here has edges)
+// and their keys never applied, so the loads need no branches and all are in
+// flight together.
+constexpr int kCmBatch = 4;
+
+template <bool W, int NT>
+__device__ __forceinline__ void cm_load_batch(const StepArgs& a, int32_t j, int32_t last, uint32_t* key,
+                                              uint32_t* f) {
+    int32_t id[kCmBatch];
+#pragma unroll
+    for (int u = 0; u < kCmBatch; ++u) id[u] = a.nbr[min(j + u * NT, last)];
+#pragma unroll
+    for (int u = 0; u < kCmBatch; ++u) key[u] = (uint32_t)a.L[id[u]] + 1u;
+#pragma unroll
+    for (int u = 0; u < kCmBatch; ++u) f[u] = W ? a.w[min(j + u * NT, last)] : 1u;
+}
+
 // Vertex verts[blockIdx.x], the sketch in LDS (NT = 64: degrees up to
-// kCountminWaveMaxDeg; NT = 1024: above).
+// kCountminWaveMaxDeg; NT = 1024: up to kCountminHubDeg).  A thread's keys
+// (at most KB of them) stay in registers from the sketch pass to the
+// estimate pass, so each neighbour label is read once.
 template <bool W, int NT>
 __global__ __launch_bounds__(NT) void k_countmin(StepArgs a, const int32_t* __restrict__ verts, int32_t rows,
                                                  int32_t width, int32_t aliased, uint64_t M) {
+    constexpr int KB = (NT == 64 ? kCountminWaveMaxDeg : kCountminHubDeg) / NT;
+    static_assert(KB % kCmBatch == 0, "whole batches");
     extern __shared__ uint32_t cells[];
     __shared__ unsigned long long s_best[NT / 64];
     const int lane = lane_id();
     const int32_t v = verts[blockIdx.x];
     const int32_t b = a.off[v], e = a.off[v + 1];
+    const int32_t last = e - 1;
     const int32_t ncell = rows * width;
     for (int32_t i = threadIdx.x; i < ncell; i += NT) cells[i] = 0u;
     __syncthreads();
-    for (int32_t j = b + (int32_t)threadIdx.x; j < e; j += NT) {
-        const uint32_t key = (uint32_t)a.L[a.nbr[j]] + 1u;
-        const uint32_t f = W ? a.w[j] : 1u;
-        uint32_t c[5];
-        const int k = cm_cells(key, rows, width, aliased, M, c);
+    uint32_t key[KB];
 #pragma unroll
-        for (int i = 0; i < 5; ++i)
-            if (i < k) atomicAdd(&cells[c[i]], f);
+    for (int q = 0; q < KB; q += kCmBatch) {
+        if (b + q * NT > last) break;  // uniform: no edges left for any thread
+        const int32_t j = b + (int32_t)threadIdx.x + q * NT;
+        uint32_t f[kCmBatch];
+        cm_load_batch<W, NT>(a, j, last, key + q, f);
+#pragma unroll
+        for (int u = 0; u < kCmBatch; ++u) {
+            if (j + u * NT > last) break;
+            uint32_t c[5];
+            const int k = cm_cells(key[q + u], rows, width, aliased, M, c);
+#pragma unroll
+            for (int i = 0; i < 5; ++i)
+                if (i < k) atomicAdd(&cells[c[i]], f[u]);
+        }
     }
     __syncthreads();
     uint64_t best = 0ull;
-    for (int32_t j = b + (int32_t)threadIdx.x; j < e; j += NT) {
-        const uint32_t key = (uint32_t)a.L[a.nbr[j]] + 1u;
-        uint32_t c[5];
-        const int k = cm_cells(key, rows, width, aliased, M, c);
-        uint32_t est = 0xffffffffu;
 #pragma unroll
-        for (int i = 0; i < 5; ++i)
-            if (i < k) est = min(est, cells[c[i]]);
-        best = max_u64(best, pack_score((int32_t)min(est, 0x7fffffffu), key));
+    for (int q = 0; q < KB; q += kCmBatch) {
+        if (b + q * NT > last) break;
+        const int32_t j = b + (int32_t)threadIdx.x + q * NT;
+#pragma unroll
+        for (int u = 0; u < kCmBatch; ++u) {
+            if (j + u * NT > last) break;
+            uint32_t c[5];
+            const int k = cm_cells(key[q + u], rows, width, aliased, M, c);
+            uint32_t est = 0xffffffffu;
+#pragma unroll
+            for (int i = 0; i < 5; ++i)
+                if (i < k) est = min(est, cells[c[i]]);
+            best = max_u64(best, pack_score((int32_t)min(est, 0x7fffffffu), key[q + u]));
+        }
     }
     best = wave_max_u64(best);
     if (NT > 64) {
@@ -209,6 +260,115 @@ __global__ __launch_bounds__(NT) void k_countmin(StepArgs a, const int32_t* __re
         const int32_t nl = best ? packed_label(best) : old;
         a.Lw[v] = nl;
         if (nl != old) atomicAdd(&a.changes[blockIdx.x & (kChangeSlots - 1)], 1ull);
+    }
+}
+
+// Count-min hubs (degree above kCountminHubDeg): workgroup i takes chunk
+// chunks[i] = (hub slot h, first edge of the chunk within the hub's edges),
+// kCountminChunk edges.  `sketch` holds rows * width cells per hub and
+// `best` one packed maximum per hub; both are zero between steps
+// (k_countmin_hub_final clears them).  Sums of cells wrap modulo 2^32 as the
+// single-workgroup form's LDS sums do, so the estimates are the same.  The
+// sketch pass leaves each edge's key in `keys` (kCountminChunk slots per
+// chunk, in edge order), which the estimate pass reads back in order instead
+// of gathering the labels again.
+template <bool W>
+__global__ __launch_bounds__(1024) void k_countmin_hub_sketch(StepArgs a, const int32_t* __restrict__ hubs,
+                                                              const int2* __restrict__ chunks,
+                                                              uint32_t* __restrict__ sketch,
+                                                              uint32_t* __restrict__ keys, int32_t rows,
+                                                              int32_t width, int32_t aliased, uint64_t M) {
+    extern __shared__ uint32_t cells[];
+    const int2 ch = chunks[blockIdx.x];
+    const int32_t v = hubs[ch.x];
+    const int32_t b = a.off[v] + ch.y, e = min(a.off[v + 1], b + kCountminChunk);
+    const int32_t last = e - 1;
+    const int32_t ncell = rows * width;
+    for (int32_t i = threadIdx.x; i < ncell; i += 1024) cells[i] = 0u;
+    __syncthreads();
+    uint32_t* ck = keys + (size_t)blockIdx.x * kCountminChunk;  // ck[j - b]: edge j's key
+    for (int32_t j = b + (int32_t)threadIdx.x; j < e; j += 1024 * kCmBatch) {
+        uint32_t key[kCmBatch], f[kCmBatch];
+        cm_load_batch<W, 1024>(a, j, last, key, f);
+#pragma unroll
+        for (int u = 0; u < kCmBatch; ++u) {
+            if (j + u * 1024 > last) break;
+            ck[j - b + u * 1024] = key[u];
+            uint32_t c[5];
+            const int k = cm_cells(key[u], rows, width, aliased, M, c);
+#pragma unroll
+            for (int i = 0; i < 5; ++i)
+                if (i < k) atomicAdd(&cells[c[i]], f[u]);
+        }
+    }
+    __syncthreads();
+    uint32_t* hs = sketch + (size_t)ch.x * ncell;
+    for (int32_t i = threadIdx.x; i < ncell; i += 1024) {
+        const uint32_t c = cells[i];
+        if (c) atomicAdd(&hs[i], c);
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_countmin_hub_est(StepArgs a, const int32_t* __restrict__ hubs,
+                                                           const int2* __restrict__ chunks,
+                                                           const uint32_t* __restrict__ sketch,
+                                                           const uint32_t* __restrict__ keys,
+                                                           unsigned long long* __restrict__ best_out, int32_t rows,
+                                                           int32_t width, int32_t aliased, uint64_t M) {
+    extern __shared__ uint32_t cells[];
+    __shared__ unsigned long long s_best[16];
+    const int lane = lane_id();
+    const int2 ch = chunks[blockIdx.x];
+    const int32_t v = hubs[ch.x];
+    const int32_t b = a.off[v] + ch.y, e = min(a.off[v + 1], b + kCountminChunk);
+    const int32_t last = e - 1;
+    const int32_t ncell = rows * width;
+    const uint32_t* hs = sketch + (size_t)ch.x * ncell;
+    for (int32_t i = threadIdx.x; i < ncell; i += 1024) cells[i] = hs[i];
+    __syncthreads();
+    const uint32_t* ck = keys + (size_t)blockIdx.x * kCountminChunk;
+    uint64_t best = 0ull;
+    for (int32_t j = b + (int32_t)threadIdx.x; j < e; j += 1024 * kCmBatch) {
+        uint32_t key[kCmBatch];
+#pragma unroll
+        for (int u = 0; u < kCmBatch; ++u) key[u] = ck[min(j + u * 1024, last) - b];
+#pragma unroll
+        for (int u = 0; u < kCmBatch; ++u) {
+            if (j + u * 1024 > last) break;
+            uint32_t c[5];
+            const int k = cm_cells(key[u], rows, width, aliased, M, c);
+            uint32_t est = 0xffffffffu;
+#pragma unroll
+            for (int i = 0; i < 5; ++i)
+                if (i < k) est = min(est, cells[c[i]]);
+            best = max_u64(best, pack_score((int32_t)min(est, 0x7fffffffu), key[u]));
+        }
+    }
+    best = wave_max_u64(best);
+    if (lane == 0) s_best[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        best = wave_max_u64(lane < 16 ? s_best[lane] : 0ull);
+        if (lane == 0 && best) atomicMax(&best_out[ch.x], (unsigned long long)best);
+    }
+}
+
+// One workgroup per count-min hub: commits its winner, then clears its
+// sketch and maximum for the next step.
+__global__ __launch_bounds__(256) void k_countmin_hub_final(StepArgs a, const int32_t* __restrict__ hubs,
+                                                            uint32_t* __restrict__ sketch,
+                                                            unsigned long long* __restrict__ best, int32_t ncell) {
+    const int32_t h = (int32_t)blockIdx.x;
+    uint32_t* hs = sketch + (size_t)h * ncell;
+    for (int32_t i = threadIdx.x; i < ncell; i += 256) hs[i] = 0u;
+    if (threadIdx.x == 0) {
+        const int32_t v = hubs[h];
+        const unsigned long long bb = best[h];
+        const int32_t old = a.keep[v];
+        const int32_t nl = bb ? packed_label(bb) : old;
+        a.Lw[v] = nl;
+        if (nl != old) atomicAdd(&a.changes[(uint32_t)h & (kChangeSlots - 1)], 1ull);
+        best[h] = 0ull;
     }
 }
 
@@ -374,30 +534,63 @@ void init_counting_kernels() {
                               lim);
     (void)hipFuncSetAttribute((const void*)k_countmin<true, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               lim);
+    (void)hipFuncSetAttribute((const void*)k_countmin_hub_sketch<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              lim);
+    (void)hipFuncSetAttribute((const void*)k_countmin_hub_sketch<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              lim);
+    (void)hipFuncSetAttribute((const void*)k_countmin_hub_est, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
 }
 
 // The counting methods' vertex lists (built once per session), in one
 // array: count-min, the vertices of degree in (kCountminSmallDeg,
-// kCountminWaveMaxDeg] (nmid of them), then those above (nwide); frequent
-// and buffer, the vertices above kTableLaneMaxDeg (nwide).
+// kCountminWaveMaxDeg] (nmid of them), then those up to kCountminHubDeg
+// (nwide), then the hubs above (ncmhub, with their chunk list, sketches and
+// maxima), then degrees 1..kCountminSmallDeg (nsmall) and 0 (niso);
+// frequent and buffer, the vertices above kTableLaneMaxDeg (nwide).
 int counting_wide_list(glp_session* s) {
     const HostCSR& g = s->g->csr;
     const bool cm = s->p.method == GLP_COUNTMIN;
     const int32_t lim = cm ? kCountminWaveMaxDeg : kTableLaneMaxDeg;
-    std::vector<int32_t> wide, big;
+    std::vector<int32_t> wide, big, hubs, iso;
+    std::vector<int32_t> small[kCountminSmallDeg + 1];  // by degree, so a wavefront's groups have alike degrees
+    std::vector<int2> chunks;
     for (int32_t v = 0; v < g.n; ++v) {
         const int32_t d = g.offsets[v + 1] - g.offsets[v];
-        if (d > lim) big.push_back(v);
+        if (cm && d > kCountminHubDeg) {
+            for (int32_t c0 = 0; c0 < d; c0 += kCountminChunk) chunks.push_back(make_int2((int32_t)hubs.size(), c0));
+            hubs.push_back(v);
+        } else if (d > lim) big.push_back(v);
         else if (cm && d > kCountminSmallDeg) wide.push_back(v);
+        else if (cm && d > 0) small[d].push_back(v);
+        else if (cm) iso.push_back(v);
     }
     s->nmid = (int32_t)wide.size();
     s->nwide = (int32_t)big.size();
+    s->ncmhub = (int32_t)hubs.size();
+    s->ncmchunk = (int32_t)chunks.size();
+    s->niso = (int32_t)iso.size();
     wide.insert(wide.end(), big.begin(), big.end());
+    wide.insert(wide.end(), hubs.begin(), hubs.end());
+    for (const auto& sd : small) wide.insert(wide.end(), sd.begin(), sd.end());
+    s->nsmall = (int32_t)(wide.size() - s->nmid - s->nwide - s->ncmhub);
+    wide.insert(wide.end(), iso.begin(), iso.end());
     if (wide.empty()) return GLP_OK;
     GLP_HIP(hipMalloc(&s->wide, wide.size() * 4));
     GLP_HIP(hipMemcpyAsync(s->wide, wide.data(), wide.size() * 4, hipMemcpyHostToDevice, s->stream));
-    GLP_HIP(hipStreamSynchronize(s->stream));  // `wide` is released on return
     s->bytes += wide.size() * 4;
+    if (!hubs.empty()) {
+        const size_t sk = (size_t)hubs.size() * s->p.cm_rows * s->p.cm_width * 4;
+        GLP_HIP(hipMalloc(&s->cm_chunks, chunks.size() * sizeof(int2)));
+        GLP_HIP(hipMalloc(&s->cm_sketch, sk));
+        GLP_HIP(hipMalloc(&s->cm_best, hubs.size() * 8));
+        GLP_HIP(hipMalloc(&s->cm_keys, chunks.size() * (size_t)kCountminChunk * 4));
+        GLP_HIP(hipMemcpyAsync(s->cm_chunks, chunks.data(), chunks.size() * sizeof(int2), hipMemcpyHostToDevice,
+                               s->stream));
+        GLP_HIP(hipMemsetAsync(s->cm_sketch, 0, sk, s->stream));
+        GLP_HIP(hipMemsetAsync(s->cm_best, 0, hubs.size() * 8, s->stream));
+        s->bytes += chunks.size() * (sizeof(int2) + (size_t)kCountminChunk * 4) + sk + hubs.size() * 8;
+    }
+    GLP_HIP(hipStreamSynchronize(s->stream));  // the host vectors are released on return
     return GLP_OK;
 }
 
@@ -419,9 +612,17 @@ int enqueue_counting_step(glp_session* s) {
             const size_t lds = (size_t)s->p.cm_rows * s->p.cm_width * 4;
             const int32_t r = s->p.cm_rows, wd = s->p.cm_width, al = s->p.cm_aliased;
             const uint64_t M = cm_mod_magic((uint32_t)wd);
-            const unsigned sgrid = (unsigned)((s->n + 15) / 16);
-            if (w) hipLaunchKernelGGL((k_countmin_small<true>), dim3(sgrid), dim3(256), 0, s->stream, a, r, wd, al, M);
-            else hipLaunchKernelGGL((k_countmin_small<false>), dim3(sgrid), dim3(256), 0, s->stream, a, r, wd, al, M);
+            const int32_t* small = s->wide + s->nmid + s->nwide + s->ncmhub;
+            if (s->nsmall > 0) {
+                const unsigned sgrid = (unsigned)((s->nsmall + 15) / 16);
+                if (w) hipLaunchKernelGGL((k_countmin_small<true>), dim3(sgrid), dim3(256), 0, s->stream, a, small,
+                                          s->nsmall, r, wd, al, M);
+                else hipLaunchKernelGGL((k_countmin_small<false>), dim3(sgrid), dim3(256), 0, s->stream, a, small,
+                                        s->nsmall, r, wd, al, M);
+            }
+            if (s->niso > 0)
+                hipLaunchKernelGGL(k_countmin_keep, dim3((unsigned)((s->niso + 255) / 256)), dim3(256), 0, s->stream,
+                                   a, small + s->nsmall, s->niso);
             if (s->nmid > 0) {
                 if (w) hipLaunchKernelGGL((k_countmin<true, 64>), dim3(s->nmid), dim3(64), lds, s->stream, a,
                                           s->wide, r, wd, al, M);
@@ -433,6 +634,20 @@ int enqueue_counting_step(glp_session* s) {
                                           a, s->wide + s->nmid, r, wd, al, M);
                 else hipLaunchKernelGGL((k_countmin<false, 1024>), dim3(s->nwide), dim3(1024), lds,
                                         s->stream, a, s->wide + s->nmid, r, wd, al, M);
+            }
+            if (s->ncmhub > 0) {
+                const int32_t* hubs = s->wide + s->nmid + s->nwide;
+                const unsigned cgrid = (unsigned)s->ncmchunk;
+                if (w)
+                    hipLaunchKernelGGL((k_countmin_hub_sketch<true>), dim3(cgrid), dim3(1024), lds, s->stream, a,
+                                       hubs, s->cm_chunks, s->cm_sketch, s->cm_keys, r, wd, al, M);
+                else
+                    hipLaunchKernelGGL((k_countmin_hub_sketch<false>), dim3(cgrid), dim3(1024), lds, s->stream, a,
+                                       hubs, s->cm_chunks, s->cm_sketch, s->cm_keys, r, wd, al, M);
+                hipLaunchKernelGGL(k_countmin_hub_est, dim3(cgrid), dim3(1024), lds, s->stream, a, hubs,
+                                   s->cm_chunks, s->cm_sketch, s->cm_keys, s->cm_best, r, wd, al, M);
+                hipLaunchKernelGGL(k_countmin_hub_final, dim3(s->ncmhub), dim3(256), 0, s->stream, a, hubs,
+                                   s->cm_sketch, s->cm_best, r * wd);
             }
         } else {
             const size_t lds = (size_t)64 * 2 * s->p.table_cap * 4;

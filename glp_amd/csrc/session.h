@@ -25,6 +25,12 @@ struct glp_session {
     int32_t* mem = nullptr;              // SLPA memory (T+1) x n
     int32_t* wide = nullptr;             // counting methods' vertex lists (count-min: nmid, then nwide)
     int32_t nmid = 0, nwide = 0;
+    int32_t ncmhub = 0, ncmchunk = 0;    // count-min hubs (after nwide in `wide`) and their chunks
+    int32_t nsmall = 0, niso = 0;        // count-min degrees 1..16 and 0 (after the hubs in `wide`)
+    int2* cm_chunks = nullptr;           // (hub slot, first edge) per chunk
+    uint32_t* cm_sketch = nullptr;       // rows x width cells per hub, zero between steps
+    unsigned long long* cm_best = nullptr;  // packed maximum per hub, zero between steps
+    uint32_t* cm_keys = nullptr;         // each chunk's edge keys, from the sketch pass to the estimate pass
     uint8_t* act[2] = {nullptr, nullptr};  // frontier flags, double-buffered like L
     int32_t* fctl = nullptr;             // frontier control {all_active, mark, threshold, gather words}
     int64_t act_bytes = 0;

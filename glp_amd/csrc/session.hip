@@ -225,6 +225,10 @@ glp_session::~glp_session() {
     (void)hipFree(d_ring);
     (void)hipFree(d_ctr);
     (void)hipFree(wide);
+    (void)hipFree(cm_chunks);
+    (void)hipFree(cm_sketch);
+    (void)hipFree(cm_best);
+    (void)hipFree(cm_keys);
     for (auto& set : ev_pool)
         for (auto& e : set)
             if (e) (void)hipEventDestroy(e);

@@ -214,6 +214,42 @@ def test_countmin_largest_sketch():
         G.run(g, 1, method=G.COUNTMIN, cm_rows=5, cm_width=4097)
 
 
+def countmin_hub_graph(weighted, seed=5):
+    """Hubs of 8192 (one chunk), 8193, 16384 and 20001 edges (a part-filled
+    last chunk) on a pool of communities, so that labels repeat among a hub's
+    neighbours from step 2 on."""
+    rng = np.random.default_rng(seed)
+    degs = [8192, 8193, 16384, 20001, 3000]
+    base, pool = len(degs), 24000
+    edges = []
+    for i, d in enumerate(degs):
+        edges += [(i, int(x)) for x in rng.choice(pool, size=d, replace=False) + base]
+    for c in range(pool // 40):
+        members = base + c * 40 + np.arange(40)
+        for _ in range(90):
+            a, b = rng.choice(members, size=2, replace=False)
+            edges.append((int(a), int(b)))
+    wts = rng.integers(1, 100, size=len(edges)).tolist() if weighted else None
+    return O.csr_from_edges(base + pool, edges, wts)
+
+
+@pytest.mark.parametrize("rows,width,aliased,weighted", [(3, 200, 1, False), (4, 50, 1, True), (1, 64, 0, False),
+                                                         (5, 4096, 0, False), (2, 7, 0, True)])
+def test_countmin_split_hubs(rows, width, aliased, weighted):
+    """Vertices above 8192 edges are counted in 8192-edge chunks on separate
+    workgroups whose sketches add up in HBM; the labels equal the oracle's,
+    which counts each vertex in one pass."""
+    off, nbr, w = countmin_hub_graph(weighted)
+    g = G.Graph.from_csr(off, nbr, w)
+    assert np.diff(off)[:5].tolist() == [8192, 8193, 16384, 20001, 3000]
+    _, dump, _ = G.run(g, 3, dump=True, method=G.COUNTMIN, cm_rows=rows, cm_width=width, cm_aliased=aliased,
+                       weighted=weighted)
+    prev = np.arange(len(off) - 1, dtype=np.int32)
+    for t in range(3):
+        prev = O.countmin_step(off, nbr, prev, rows, width, aliased, w=w if weighted else None)
+        assert np.array_equal(dump[t], prev), "step %d" % (t + 1)
+
+
 def test_countmin_weighted(rmat13w):
     g, (off, nbr, w) = rmat13w
     labels, _ = G.run(g, 2, method=G.COUNTMIN, cm_rows=3, cm_width=100, cm_aliased=1, weighted=True)
