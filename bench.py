@@ -667,8 +667,9 @@ def main():
     ap.add_argument("--scale", type=int, default=25)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--method", choices=["classic", "layered"], default="classic",
-                    help="layered: the headline leg runs layered LP with --gamma (profiling runs)")
+    ap.add_argument("--method", choices=["classic", "layered", "counting"], default="classic",
+                    help="layered: the headline leg runs layered LP with --gamma; counting: only the count-min, "
+                         "frequent and buffer legs (profiling runs)")
     ap.add_argument("--gamma", default="1/4", help="layered gamma p/q (q a power of two)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -742,6 +743,10 @@ def main():
     n, m = info["n"], info["m"]
     log("graph n=%d m=%d max_degree=%d" % (n, m, info["max_degree"]))
     wl_key = "rmat-s%d-ef%d-seed%d-classic" % (args.scale, args.edge_factor, args.seed)
+
+    if args.method == "counting":
+        print(json.dumps({"counting": counting_legs(args, G, g, m)}), flush=True)
+        return
 
     if args.method == "layered":
         # profiling form: the headline leg runs layered LP (gamma p/q)
