@@ -410,6 +410,7 @@ __global__ __launch_bounds__(256) void k_table_wave(StepArgs a, const int32_t* _
     const int32_t b = __builtin_amdgcn_readfirstlane(a.off[v]);
     const int32_t e = __builtin_amdgcn_readfirstlane(a.off[v + 1]);
     const bool slot = lane < cap;
+    const uint64_t slots = __ballot(slot);
     uint32_t K = slot ? 0u : 0xffffffffu, C = 0u;
     const int32_t last = e - 1;  // e > b: these vertices have more than 32 edges
     int32_t jn = b + lane;
@@ -423,7 +424,25 @@ __global__ __launch_bounds__(256) void k_table_wave(StepArgs a, const int32_t* _
         const uint32_t nf = W ? a.w[min(jn, last)] : 1u;
         jn += 64;
         id = a.nbr[min(jn, last)];
-        const int32_t cnt = min(64, e - j0);
+        const int32_t cnt = __builtin_amdgcn_readfirstlane(min(64, e - j0));  // held in a scalar register
+        if constexpr (!W) {
+            // Unweighted, without branches: the key's slot (the one holding
+            // it, else the lowest free one) gains one and holds it, else
+            // (full) every slot drops by one and the emptied ones are freed.
+            // Frequent and buffer agree here (every count >= 1, frequency 1).
+            // The two slot sets are wave-uniform masks, read back per lane.
+            for (int32_t k = 0; k < cnt; ++k) {
+                const uint32_t kk = (uint32_t)__builtin_amdgcn_readlane((int)lab, k) + 1u;
+                const uint64_t hm = __ballot(K == kk), fm = __ballot(K == 0u);
+                const uint64_t upm = hm ? hm : fm & (0ull - fm);
+                const uint64_t dnm = hm | fm ? 0ull : slots;
+                const bool up = __builtin_amdgcn_inverse_ballot_w64(upm);
+                const bool dn = __builtin_amdgcn_inverse_ballot_w64(dnm);
+                C = C + (uint32_t)up - (uint32_t)dn;
+                K = up ? kk : K;
+                K = dn && C == 0u ? 0u : K;
+            }
+        } else
         for (int32_t k = 0; k < cnt; ++k) {
             const uint32_t kk = (uint32_t)__builtin_amdgcn_readlane((int)lab, k) + 1u;
             const uint32_t ff = W ? (uint32_t)__builtin_amdgcn_readlane((int)f, k) : 1u;
