@@ -493,45 +493,49 @@ __global__ __launch_bounds__(256) void k_table_wave(StepArgs a, const int32_t* _
 }
 
 // One lane per vertex of degree <= kTableLaneMaxDeg; the lane's table lives
-// in LDS (C key and C count slots per lane).
+// in LDS (cap key and cap count slots per lane), slot-major (slot i of lane l
+// at word i * 64 + l), so the lanes' accesses to one slot fall in different
+// banks.  The next edge's label loads while an edge is applied.
 template <bool W, bool BUF>
 __global__ __launch_bounds__(64) void k_table(StepArgs a, int32_t cap) {
     extern __shared__ uint32_t tbl[];
     const int lane = lane_id();
     const int32_t v = (int32_t)(blockIdx.x * 64 + lane);
-    uint32_t* K = tbl + lane * 2 * cap;
-    uint32_t* C = K + cap;
+    uint32_t* const K = tbl + lane;             // K[i * 64]: slot i's key
+    uint32_t* const C = tbl + cap * 64 + lane;  // C[i * 64]: slot i's count
     bool changed = false;
     if (v < a.n && a.off[v + 1] - a.off[v] <= kTableLaneMaxDeg) {
-        for (int i = 0; i < cap; ++i) { K[i] = 0u; C[i] = 0u; }
+        for (int i = 0; i < cap; ++i) { K[i * 64] = 0u; C[i * 64] = 0u; }
         const int32_t b = a.off[v], e = a.off[v + 1];
+        uint32_t nkey = b < e ? (uint32_t)a.L[a.nbr[b]] + 1u : 0u;
         for (int32_t j = b; j < e; ++j) {
-            const uint32_t key = (uint32_t)a.L[a.nbr[j]] + 1u;
+            const uint32_t key = nkey;
+            if (j + 1 < e) nkey = (uint32_t)a.L[a.nbr[j + 1]] + 1u;
             const uint32_t f = W ? a.w[j] : 1u;
             int hit = -1, fr = -1;
             uint32_t cmin = 0xffffffffu;
             for (int i = 0; i < cap; ++i) {
-                const uint32_t k = K[i];
+                const uint32_t k = K[i * 64];
                 if (k == key) hit = i;
                 if (k == 0u) { if (fr < 0) fr = i; }
-                else cmin = min(cmin, C[i]);
+                else cmin = min(cmin, C[i * 64]);
             }
-            if (hit >= 0) { C[hit] += f; continue; }
-            if (fr >= 0) { K[fr] = key; C[fr] = f; continue; }
+            if (hit >= 0) { C[hit * 64] += f; continue; }
+            if (fr >= 0) { K[fr * 64] = key; C[fr * 64] = f; continue; }
             // table full
             const uint32_t d = min(f, cmin);
             int freed = -1;
             for (int i = 0; i < cap; ++i) {
-                const uint32_t c = C[i];
+                const uint32_t c = C[i * 64];
                 const uint32_t nc = BUF ? (c > f ? c - f : 0u) : c - d;
-                C[i] = nc;
-                if (nc == 0u) { K[i] = 0u; if (freed < 0) freed = i; }
+                C[i * 64] = nc;
+                if (nc == 0u) { K[i * 64] = 0u; if (freed < 0) freed = i; }
             }
-            if (!BUF && f > d) { K[freed] = key; C[freed] = f - d; }
+            if (!BUF && f > d) { K[freed * 64] = key; C[freed * 64] = f - d; }
         }
         uint64_t best = 0ull;
         for (int i = 0; i < cap; ++i)
-            if (K[i]) best = max_u64(best, pack_score((int32_t)min(C[i], 0x7fffffffu), K[i]));
+            if (K[i * 64]) best = max_u64(best, pack_score((int32_t)min(C[i * 64], 0x7fffffffu), K[i * 64]));
         const int32_t old = a.keep[v];
         const int32_t nl = best ? packed_label(best) : old;
         a.Lw[v] = nl;
