@@ -97,6 +97,7 @@ __device__ __forceinline__ int cm_cells(uint32_t key, int rows, int W, int alias
 
 constexpr int32_t kCountminSmallDeg = 16;
 constexpr int32_t kCountminWaveMaxDeg = 1024;
+constexpr int32_t kCountminMidDeg = 4096;  // up to here 256 threads per vertex, then 1024
 constexpr int32_t kCountminHubDeg = 8192;
 constexpr int32_t kCountminChunk = 8192;  // 1024 threads x 8 edges
 constexpr int32_t kTableLaneMaxDeg = 32;
@@ -196,13 +197,15 @@ __device__ __forceinline__ void cm_load_batch(const StepArgs& a, int32_t j, int3
 }
 
 // Vertex verts[blockIdx.x], the sketch in LDS (NT = 64: degrees up to
-// kCountminWaveMaxDeg; NT = 1024: up to kCountminHubDeg).  A thread's keys
-// (at most KB of them) stay in registers from the sketch pass to the
-// estimate pass, so each neighbour label is read once.
+// kCountminWaveMaxDeg; NT = 256: up to kCountminMidDeg; NT = 1024: up to
+// kCountminHubDeg).  A thread's keys (at most KB of them) stay in registers
+// from the sketch pass to the estimate pass, so each neighbour label is read
+// once.
 template <bool W, int NT>
 __global__ __launch_bounds__(NT) void k_countmin(StepArgs a, const int32_t* __restrict__ verts, int32_t rows,
                                                  int32_t width, int32_t aliased, uint64_t M) {
-    constexpr int KB = (NT == 64 ? kCountminWaveMaxDeg : kCountminHubDeg) / NT;
+    constexpr int KB =
+        (NT == 64 ? kCountminWaveMaxDeg : NT == 256 ? kCountminMidDeg : kCountminHubDeg) / NT;
     static_assert(KB % kCmBatch == 0, "whole batches");
     extern __shared__ uint32_t cells[];
     __shared__ unsigned long long s_best[NT / 64];
@@ -553,6 +556,9 @@ void init_counting_kernels() {
     const int lim = 5 * 4096 * 4;
     (void)hipFuncSetAttribute((const void*)k_countmin<false, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     (void)hipFuncSetAttribute((const void*)k_countmin<true, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void*)k_countmin<false, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              lim);
+    (void)hipFuncSetAttribute((const void*)k_countmin<true, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     (void)hipFuncSetAttribute((const void*)k_countmin<false, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               lim);
     (void)hipFuncSetAttribute((const void*)k_countmin<true, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -567,14 +573,14 @@ void init_counting_kernels() {
 // The counting methods' vertex lists (built once per session), in one
 // array: count-min, the vertices of degree in (kCountminSmallDeg,
 // kCountminWaveMaxDeg] (nmid of them), then those up to kCountminHubDeg
-// (nwide), then the hubs above (ncmhub, with their chunk list, sketches and
+// (nwide: the ncm256 up to kCountminMidDeg first), then the hubs above (ncmhub, with their chunk list, sketches and
 // maxima), then degrees 1..kCountminSmallDeg (nsmall) and 0 (niso);
 // frequent and buffer, the vertices above kTableLaneMaxDeg (nwide).
 int counting_wide_list(glp_session* s) {
     const HostCSR& g = s->g->csr;
     const bool cm = s->p.method == GLP_COUNTMIN;
     const int32_t lim = cm ? kCountminWaveMaxDeg : kTableLaneMaxDeg;
-    std::vector<int32_t> wide, big, hubs, iso;
+    std::vector<int32_t> wide, big, big1024, hubs, iso;
     std::vector<int32_t> small[kCountminSmallDeg + 1];  // by degree, so a wavefront's groups have alike degrees
     std::vector<int2> chunks;
     for (int32_t v = 0; v < g.n; ++v) {
@@ -582,12 +588,15 @@ int counting_wide_list(glp_session* s) {
         if (cm && d > kCountminHubDeg) {
             for (int32_t c0 = 0; c0 < d; c0 += kCountminChunk) chunks.push_back(make_int2((int32_t)hubs.size(), c0));
             hubs.push_back(v);
-        } else if (d > lim) big.push_back(v);
+        } else if (cm && d > kCountminMidDeg) big1024.push_back(v);
+        else if (d > lim) big.push_back(v);
         else if (cm && d > kCountminSmallDeg) wide.push_back(v);
         else if (cm && d > 0) small[d].push_back(v);
         else if (cm) iso.push_back(v);
     }
     s->nmid = (int32_t)wide.size();
+    s->ncm256 = (int32_t)big.size();
+    big.insert(big.end(), big1024.begin(), big1024.end());
     s->nwide = (int32_t)big.size();
     s->ncmhub = (int32_t)hubs.size();
     s->ncmchunk = (int32_t)chunks.size();
@@ -652,11 +661,19 @@ int enqueue_counting_step(glp_session* s) {
                 else hipLaunchKernelGGL((k_countmin<false, 64>), dim3(s->nmid), dim3(64), lds, s->stream, a,
                                         s->wide, r, wd, al, M);
             }
-            if (s->nwide > 0) {
-                if (w) hipLaunchKernelGGL((k_countmin<true, 1024>), dim3(s->nwide), dim3(1024), lds, s->stream,
+            if (s->ncm256 > 0) {
+                if (w) hipLaunchKernelGGL((k_countmin<true, 256>), dim3(s->ncm256), dim3(256), lds, s->stream,
                                           a, s->wide + s->nmid, r, wd, al, M);
-                else hipLaunchKernelGGL((k_countmin<false, 1024>), dim3(s->nwide), dim3(1024), lds,
-                                        s->stream, a, s->wide + s->nmid, r, wd, al, M);
+                else hipLaunchKernelGGL((k_countmin<false, 256>), dim3(s->ncm256), dim3(256), lds, s->stream,
+                                        a, s->wide + s->nmid, r, wd, al, M);
+            }
+            if (s->nwide > s->ncm256) {
+                const unsigned g1024 = (unsigned)(s->nwide - s->ncm256);
+                const int32_t* v1024 = s->wide + s->nmid + s->ncm256;
+                if (w) hipLaunchKernelGGL((k_countmin<true, 1024>), dim3(g1024), dim3(1024), lds, s->stream,
+                                          a, v1024, r, wd, al, M);
+                else hipLaunchKernelGGL((k_countmin<false, 1024>), dim3(g1024), dim3(1024), lds, s->stream,
+                                        a, v1024, r, wd, al, M);
             }
             if (s->ncmhub > 0) {
                 const int32_t* hubs = s->wide + s->nmid + s->nwide;
