@@ -214,12 +214,12 @@ def test_countmin_largest_sketch():
         G.run(g, 1, method=G.COUNTMIN, cm_rows=5, cm_width=4097)
 
 
-def countmin_hub_graph(weighted, seed=5):
+def countmin_hub_graph(weighted, seed=5, degs=(8192, 8193, 16384, 20001, 3000)):
     """Hubs of 8192 (one chunk), 8193, 16384 and 20001 edges (a part-filled
     last chunk) on a pool of communities, so that labels repeat among a hub's
     neighbours from step 2 on."""
     rng = np.random.default_rng(seed)
-    degs = [8192, 8193, 16384, 20001, 3000]
+    degs = list(degs)
     base, pool = len(degs), 24000
     edges = []
     for i, d in enumerate(degs):
@@ -247,6 +247,28 @@ def test_countmin_split_hubs(rows, width, aliased, weighted):
     prev = np.arange(len(off) - 1, dtype=np.int32)
     for t in range(3):
         prev = O.countmin_step(off, nbr, prev, rows, width, aliased, w=w if weighted else None)
+        assert np.array_equal(dump[t], prev), "step %d" % (t + 1)
+
+
+TIER_EDGES = (16, 17, 32, 33, 1024, 1025, 4096, 4097, 8192, 8193)
+
+
+@pytest.mark.parametrize("method", [G.COUNTMIN, G.FREQUENT, G.BUFFER])
+def test_counting_tier_boundaries(method):
+    """Centres with degrees on each side of every counting tier's bound
+    (count-min 16 / 1024 / 4096 / 8192, tables 32): a thread's keys held in
+    registers must cover its tier's largest degree exactly."""
+    off, nbr, _ = countmin_hub_graph(False, seed=9, degs=TIER_EDGES)
+    assert np.diff(off)[:len(TIER_EDGES)].tolist() == list(TIER_EDGES)
+    g = G.Graph.from_csr(off, nbr)
+    kw = dict(cm_rows=4, cm_width=50, cm_aliased=1) if method == G.COUNTMIN else dict(table_cap=8)
+    _, dump, _ = G.run(g, 3, dump=True, method=method, **kw)
+    prev = np.arange(len(off) - 1, dtype=np.int32)
+    for t in range(3):
+        if method == G.COUNTMIN:
+            prev = O.countmin_step(off, nbr, prev, 4, 50, 1)
+        else:
+            prev = O.table_step(off, nbr, prev, 8, int(method == G.BUFFER))
         assert np.array_equal(dump[t], prev), "step %d" % (t + 1)
 
 
