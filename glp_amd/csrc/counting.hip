@@ -434,8 +434,9 @@ __global__ __launch_bounds__(256) void k_table_wave(StepArgs a, const int32_t* _
             // (full) every slot drops by one and the emptied ones are freed.
             // Frequent and buffer agree here (every count >= 1, frequency 1).
             // The two slot sets are wave-uniform masks, read back per lane.
-            for (int32_t k = 0; k < cnt; ++k) {
-                const uint32_t kk = (uint32_t)__builtin_amdgcn_readlane((int)lab, k) + 1u;
+            // A full chunk runs unrolled (no loop control per edge).
+            const uint32_t key = lab + 1u;
+            auto apply = [&](uint32_t kk) {
                 const uint64_t hm = __ballot(K == kk), fm = __ballot(K == 0u);
                 const uint64_t upm = hm ? hm : fm & (0ull - fm);
                 const uint64_t dnm = hm | fm ? 0ull : slots;
@@ -444,6 +445,12 @@ __global__ __launch_bounds__(256) void k_table_wave(StepArgs a, const int32_t* _
                 C = C + (uint32_t)up - (uint32_t)dn;
                 K = up ? kk : K;
                 K = dn && C == 0u ? 0u : K;
+            };
+            if (cnt == 64) {
+#pragma unroll
+                for (int k = 0; k < 64; ++k) apply((uint32_t)__builtin_amdgcn_readlane((int)key, k));
+            } else {
+                for (int32_t k = 0; k < cnt; ++k) apply((uint32_t)__builtin_amdgcn_readlane((int)key, k));
             }
         } else
         for (int32_t k = 0; k < cnt; ++k) {
