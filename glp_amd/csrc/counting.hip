@@ -582,7 +582,9 @@ void init_counting_kernels() {
 // kCountminWaveMaxDeg] (nmid of them), then those up to kCountminHubDeg
 // (nwide: the ncm256 up to kCountminMidDeg first), then the hubs above (ncmhub, with their chunk list, sketches and
 // maxima), then degrees 1..kCountminSmallDeg (nsmall) and 0 (niso);
-// frequent and buffer, the vertices above kTableLaneMaxDeg (nwide).
+// frequent and buffer, the vertices above kTableLaneMaxDeg (nwide).  The
+// lists of one workgroup or wavefront per vertex, and the hubs, run largest
+// degree first, so the longest counts start at once whatever the vertex ids.
 int counting_wide_list(glp_session* s) {
     const HostCSR& g = s->g->csr;
     const bool cm = s->p.method == GLP_COUNTMIN;
@@ -592,15 +594,18 @@ int counting_wide_list(glp_session* s) {
     std::vector<int2> chunks;
     for (int32_t v = 0; v < g.n; ++v) {
         const int32_t d = g.offsets[v + 1] - g.offsets[v];
-        if (cm && d > kCountminHubDeg) {
-            for (int32_t c0 = 0; c0 < d; c0 += kCountminChunk) chunks.push_back(make_int2((int32_t)hubs.size(), c0));
-            hubs.push_back(v);
-        } else if (cm && d > kCountminMidDeg) big1024.push_back(v);
+        if (cm && d > kCountminHubDeg) hubs.push_back(v);
+        else if (cm && d > kCountminMidDeg) big1024.push_back(v);
         else if (d > lim) big.push_back(v);
         else if (cm && d > kCountminSmallDeg) wide.push_back(v);
         else if (cm && d > 0) small[d].push_back(v);
         else if (cm) iso.push_back(v);
     }
+    auto deg = [&g](int32_t v) { return g.offsets[v + 1] - g.offsets[v]; };
+    for (auto* list : {&wide, &big, &big1024, &hubs})
+        std::stable_sort(list->begin(), list->end(), [&](int32_t x, int32_t y) { return deg(x) > deg(y); });
+    for (int32_t h = 0; h < (int32_t)hubs.size(); ++h)
+        for (int32_t c0 = 0; c0 < deg(hubs[h]); c0 += kCountminChunk) chunks.push_back(make_int2(h, c0));
     s->nmid = (int32_t)wide.size();
     s->ncm256 = (int32_t)big.size();
     big.insert(big.end(), big1024.begin(), big1024.end());
