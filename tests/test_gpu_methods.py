@@ -272,6 +272,25 @@ def test_counting_tier_boundaries(method):
         assert np.array_equal(dump[t], prev), "step %d" % (t + 1)
 
 
+@pytest.mark.parametrize("method", [G.COUNTMIN, G.FREQUENT, G.BUFFER])
+def test_counting_tiny_and_edgeless_graphs(method):
+    """A single vertex, edge-less graphs, one edge and a 4-cycle: the
+    counting methods' vertex lists are empty or hold only isolated vertices,
+    and every label matches the oracle (an edge-less vertex keeps its own)."""
+    kw = dict(cm_rows=3, cm_width=200, cm_aliased=1) if method == G.COUNTMIN else dict(table_cap=4)
+    for n, edges in ((1, []), (3, []), (2, [(0, 1)]), (4, [(0, 1), (1, 2), (2, 3), (3, 0)])):
+        g = G.Graph.from_edges(n, edges)
+        off, nbr, _ = g.csr()
+        labels, _ = G.run(g, 3, method=method, **kw)
+        prev = np.arange(n, dtype=np.int32)
+        for _ in range(3):
+            if method == G.COUNTMIN:
+                prev = O.countmin_step(off, nbr, prev, 3, 200, 1)
+            else:
+                prev = O.table_step(off, nbr, prev, 4, int(method == G.BUFFER))
+        assert np.array_equal(labels, prev), (n, edges)
+
+
 def test_countmin_weighted(rmat13w):
     g, (off, nbr, w) = rmat13w
     labels, _ = G.run(g, 2, method=G.COUNTMIN, cm_rows=3, cm_width=100, cm_aliased=1, weighted=True)
