@@ -452,41 +452,36 @@ __global__ __launch_bounds__(256) void k_table_wave(StepArgs a, const int32_t* _
             } else {
                 for (int32_t k = 0; k < cnt; ++k) apply((uint32_t)__builtin_amdgcn_readlane((int)key, k));
             }
-        } else
-        for (int32_t k = 0; k < cnt; ++k) {
-            const uint32_t kk = (uint32_t)__builtin_amdgcn_readlane((int)lab, k) + 1u;
-            const uint32_t ff = W ? (uint32_t)__builtin_amdgcn_readlane((int)f, k) : 1u;
-            const bool hit = K == kk;
-            if (__ballot(hit)) {  // present: it gains the frequency
-                if (hit) C += ff;
-                continue;
-            }
-            const uint64_t fr = __ballot(K == 0u);
-            if (fr) {  // the first free slot takes it
-                if (lane == __builtin_ctzll(fr)) {
-                    K = kk;
-                    C = ff;
+        } else {
+            for (int32_t k = 0; k < cnt; ++k) {
+                const uint32_t kk = (uint32_t)__builtin_amdgcn_readlane((int)lab, k) + 1u;
+                const uint32_t ff = (uint32_t)__builtin_amdgcn_readlane((int)f, k);
+                const bool hit = K == kk;
+                if (__ballot(hit)) {  // present: it gains the frequency
+                    if (hit) C += ff;
+                    continue;
                 }
-                continue;
-            }
-            // full: every counter drops (buffer: by the frequency; frequent:
-            // by min(frequency, smallest counter)), emptied slots are freed.
-            // Unweighted, every count is >= 1 and the frequency 1, so both
-            // drop by one and nothing is left to enter.
-            if (!W) {
-                if (slot) C -= 1u;
-                if (slot && C == 0u) K = 0u;
-                continue;
-            }
-            uint32_t d = ff;
-            if (!BUF) d = min(ff, wave_min_dpp(slot ? C : 0xffffffffu));
-            if (slot) C = BUF ? (C > ff ? C - ff : 0u) : C - d;
-            const bool emptied = slot && C == 0u;
-            if (emptied) K = 0u;
-            if (!BUF && ff > d) {  // frequent: what is left enters the first freed slot
-                if (lane == __builtin_ctzll(__ballot(emptied))) {
-                    K = kk;
-                    C = ff - d;
+                const uint64_t fr = __ballot(K == 0u);
+                if (fr) {  // the first free slot takes it
+                    if (lane == __builtin_ctzll(fr)) {
+                        K = kk;
+                        C = ff;
+                    }
+                    continue;
+                }
+                // full: every counter drops (buffer: by the frequency;
+                // frequent: by min(frequency, smallest counter)), emptied
+                // slots are freed
+                uint32_t d = ff;
+                if (!BUF) d = min(ff, wave_min_dpp(slot ? C : 0xffffffffu));
+                if (slot) C = BUF ? (C > ff ? C - ff : 0u) : C - d;
+                const bool emptied = slot && C == 0u;
+                if (emptied) K = 0u;
+                if (!BUF && ff > d) {  // frequent: what is left enters the first freed slot
+                    if (lane == __builtin_ctzll(__ballot(emptied))) {
+                        K = kk;
+                        C = ff - d;
+                    }
                 }
             }
         }
