@@ -96,22 +96,26 @@ __device__ __forceinline__ int cm_cells(uint32_t key, int rows, int W, int alias
 }
 
 constexpr int32_t kCountminSmallDeg = 16;
+constexpr int kCmSmallClasses = 5;  // lane groups of 1, 2, 4, 8, 16 for degrees 1, 2, 3-4, 5-8, 9-16
+static_assert(1 << (kCmSmallClasses - 1) == kCountminSmallDeg, "the last class reaches kCountminSmallDeg");
 constexpr int32_t kCountminWaveMaxDeg = 1024;
 constexpr int32_t kCountminMidDeg = 4096;  // up to here 256 threads per vertex, then 1024
 constexpr int32_t kCountminHubDeg = 8192;
 constexpr int32_t kCountminChunk = 8192;  // 1024 threads x 8 edges
 constexpr int32_t kTableLaneMaxDeg = 32;
 
-// The vertices of `verts` (degrees 1..kCountminSmallDeg), 16 lanes each
-// (lane i holds edge i).  The sketch cell an entry c of edge i lands in ends
-// up holding the sum, over the vertex's edges j and their entries, of f_j
-// where the entry is c; the estimate is the minimum of that over edge i's
-// entries.
-template <bool W>
+// The vertices of `verts` (degrees 1..G), G lanes each (lane i holds edge
+// i; G = 1, 2, 4, 8 or 16, so 64 / G vertices share a wavefront).  The
+// sketch cell an entry c of edge i lands in ends up holding the sum, over
+// the vertex's edges j and their entries, of f_j where the entry is c; the
+// estimate is the minimum of that over edge i's entries.
+template <bool W, int G>
 __global__ __launch_bounds__(256) void k_countmin_small(StepArgs a, const int32_t* __restrict__ verts, int32_t nv,
                                                        int32_t rows, int32_t width, int32_t aliased, uint64_t M) {
-    const int lane = lane_id(), ql = lane & 15, g0 = lane & ~15;
-    const int64_t vv = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
+    constexpr int kShift = G == 1 ? 0 : G == 2 ? 1 : G == 4 ? 2 : G == 8 ? 3 : 4;
+    static_assert((1 << kShift) == G && G <= kCountminSmallDeg, "group of 1, 2, 4, 8 or 16 lanes");
+    const int lane = lane_id(), ql = lane & (G - 1), g0 = lane & ~(G - 1);
+    const int64_t vv = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> kShift;
     const bool in = vv < nv;
     const int32_t v = in ? verts[vv] : 0;
     int32_t b = 0, e = 0;
@@ -156,7 +160,8 @@ __global__ __launch_bounds__(256) void k_countmin_small(StepArgs a, const int32_
     for (int t = 0; t < 5; ++t)
         if (t < k) est = min(est, sum[t]);
     uint64_t best = live ? pack_score((int32_t)min(est, 0x7fffffffu), key) : 0ull;
-    for (int d = 8; d >= 1; d >>= 1) best = max_u64(best, shfl_u64(best, lane ^ d));
+#pragma unroll
+    for (int d = G / 2; d >= 1; d >>= 1) best = max_u64(best, shfl_u64(best, lane ^ d));
     bool changed = false;
     if (in && ql == 0) {
         const int32_t old = a.keep[v];
@@ -167,6 +172,28 @@ __global__ __launch_bounds__(256) void k_countmin_small(StepArgs a, const int32_
     const unsigned long long bal = __ballot(changed);
     if (lane == 0 && bal)
         atomicAdd(&a.changes[blockIdx.x & (kChangeSlots - 1)], (unsigned long long)__popcll(bal));
+}
+
+// One launch of k_countmin_small over n listed vertices of lane-group class
+// gi (groups of 1 << gi lanes).
+template <int G>
+static void launch_countmin_small_g(bool w, const StepArgs& a, const int32_t* verts, int32_t n, int32_t rows,
+                                    int32_t width, int32_t aliased, uint64_t M, hipStream_t s) {
+    const unsigned grid = (unsigned)(((int64_t)n * G + 255) / 256);
+    if (w) hipLaunchKernelGGL((k_countmin_small<true, G>), dim3(grid), dim3(256), 0, s, a, verts, n, rows, width,
+                              aliased, M);
+    else hipLaunchKernelGGL((k_countmin_small<false, G>), dim3(grid), dim3(256), 0, s, a, verts, n, rows, width,
+                            aliased, M);
+}
+static void launch_countmin_small(int gi, bool w, const StepArgs& a, const int32_t* verts, int32_t n, int32_t rows,
+                                  int32_t width, int32_t aliased, uint64_t M, hipStream_t s) {
+    switch (gi) {
+        case 0: launch_countmin_small_g<1>(w, a, verts, n, rows, width, aliased, M, s); break;
+        case 1: launch_countmin_small_g<2>(w, a, verts, n, rows, width, aliased, M, s); break;
+        case 2: launch_countmin_small_g<4>(w, a, verts, n, rows, width, aliased, M, s); break;
+        case 3: launch_countmin_small_g<8>(w, a, verts, n, rows, width, aliased, M, s); break;
+        default: launch_countmin_small_g<16>(w, a, verts, n, rows, width, aliased, M, s); break;
+    }
 }
 
 // Vertices without edges keep their label.
@@ -612,6 +639,11 @@ int counting_wide_list(glp_session* s) {
     wide.insert(wide.end(), hubs.begin(), hubs.end());
     for (const auto& sd : small) wide.insert(wide.end(), sd.begin(), sd.end());
     s->nsmall = (int32_t)(wide.size() - s->nmid - s->nwide - s->ncmhub);
+    // the small list's lane-group classes: degree 1, 2, 3-4, 5-8, 9-16
+    for (int gi = 0, d = 1; gi < kCmSmallClasses; ++gi) {
+        s->nsmall_g[gi] = 0;
+        for (; d <= (1 << gi); ++d) s->nsmall_g[gi] += (int32_t)small[d].size();
+    }
     wide.insert(wide.end(), iso.begin(), iso.end());
     if (wide.empty()) return GLP_OK;
     GLP_HIP(hipMalloc(&s->wide, wide.size() * 4));
@@ -652,13 +684,9 @@ int enqueue_counting_step(glp_session* s) {
             const int32_t r = s->p.cm_rows, wd = s->p.cm_width, al = s->p.cm_aliased;
             const uint64_t M = cm_mod_magic((uint32_t)wd);
             const int32_t* small = s->wide + s->nmid + s->nwide + s->ncmhub;
-            if (s->nsmall > 0) {
-                const unsigned sgrid = (unsigned)((s->nsmall + 15) / 16);
-                if (w) hipLaunchKernelGGL((k_countmin_small<true>), dim3(sgrid), dim3(256), 0, s->stream, a, small,
-                                          s->nsmall, r, wd, al, M);
-                else hipLaunchKernelGGL((k_countmin_small<false>), dim3(sgrid), dim3(256), 0, s->stream, a, small,
-                                        s->nsmall, r, wd, al, M);
-            }
+            for (int gi = 0, at = 0; gi < kCmSmallClasses; at += s->nsmall_g[gi], ++gi)
+                if (s->nsmall_g[gi] > 0)
+                    launch_countmin_small(gi, w, a, small + at, s->nsmall_g[gi], r, wd, al, M, s->stream);
             if (s->niso > 0)
                 hipLaunchKernelGGL(k_countmin_keep, dim3((unsigned)((s->niso + 255) / 256)), dim3(256), 0, s->stream,
                                    a, small + s->nsmall, s->niso);

@@ -28,6 +28,7 @@ struct glp_session {
     int32_t ncmhub = 0, ncmchunk = 0;    // count-min hubs (after nwide in `wide`) and their chunks
     int32_t ncm256 = 0;                  // count-min: the first ncm256 of nwide take 256-thread workgroups
     int32_t nsmall = 0, niso = 0;        // count-min degrees 1..16 and 0 (after the hubs in `wide`)
+    int32_t nsmall_g[5] = {0, 0, 0, 0, 0};  // the small list's degree classes 1, 2, 3-4, 5-8, 9-16
     int2* cm_chunks = nullptr;           // (hub slot, first edge) per chunk
     uint32_t* cm_sketch = nullptr;       // rows x width cells per hub, zero between steps
     unsigned long long* cm_best = nullptr;  // packed maximum per hub, zero between steps
